@@ -1,0 +1,352 @@
+"""hfscf_amd — MI355X-native RHF two-electron engine behind hfscf's seams.
+
+Python bindings over two C ABIs:
+  * ``include/hfgpu.h``  (lib/libhfgpu.so)  the GPU engine: Schwarz screening, shell-quartet
+    ERIs, direct J/K / Fock builds and the adjoint two-electron gradient (gfx950 HIP kernels);
+  * ``include/hfscf.h``  (lib/libhfscf.so)  the host RHF layer: basis and molecule handling,
+    one-electron integrals, DIIS, the SCF loop and gradient assembly.
+
+The names follow hfscf: ``Molecule`` (MOLEC::Molecule), ``rhf`` (Mol::hf_solver::scf_run),
+``rhf_gradient`` (Mol::scf_gradient::calc_scf_gradient_rhf), ``GpuContext`` (the Erios /
+update_fock_matrix replacement). There is no CPU fallback: without the built libraries or a
+GPU every call raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBDIR = os.path.join(HERE, "lib")
+BASISDIR = os.path.join(HERE, "basis")
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+FP64, MIXED = 0, 1
+
+
+class HfgOptions(C.Structure):
+    _fields_ = [("precision", C.c_int), ("integral_tol", C.c_double), ("schwarz", C.c_int),
+                ("reserved", C.c_int * 8)]
+
+
+class HfgStats(C.Structure):
+    _fields_ = [("quartets", C.c_longlong), ("model_flops", C.c_double), ("kernel_seconds", C.c_double),
+                ("total_seconds", C.c_double), ("work_items", C.c_longlong), ("slab_bytes", C.c_longlong)]
+
+
+class ScfOptions(C.Structure):
+    _fields_ = [("backend", C.c_int), ("device", C.c_int), ("precision", C.c_int), ("incremental", C.c_int),
+                ("diis_range", C.c_int), ("max_iter", C.c_int), ("schwarz", C.c_int), ("verbose", C.c_int),
+                ("energy_tol", C.c_double), ("rms_tol", C.c_double), ("integral_tol", C.c_double),
+                ("final_fp64", C.c_int), ("reserved", C.c_int * 7)]
+
+
+class ScfResult(C.Structure):
+    _fields_ = [("energy", C.c_double), ("energy_elec", C.c_double), ("energy_nuc", C.c_double),
+                ("iterations", C.c_int), ("converged", C.c_int), ("fock_seconds", C.c_double),
+                ("last_fock_seconds", C.c_double), ("fock_builds", C.c_int)]
+
+
+JKFUNC = C.CFUNCTYPE(C.c_int, C.c_void_p, _dp, _dp, _dp)
+GRADFUNC = C.CFUNCTYPE(C.c_int, C.c_void_p, _dp, _dp)
+
+
+class Callbacks(C.Structure):
+    _fields_ = [("user", C.c_void_p), ("build_jk", JKFUNC), ("grad2e", GRADFUNC)]
+
+
+_libs = {}
+
+
+def _load(name):
+    if name not in _libs:
+        path = os.path.join(LIBDIR, name)
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} is not built: run __graft_entry__.build() (make -C hfscf_amd/csrc)")
+        _libs[name] = C.CDLL(path)
+    return _libs[name]
+
+
+def hfgpu():
+    """ctypes handle of libhfgpu.so with argument types declared."""
+    lib = _load("libhfgpu.so")
+    if not getattr(lib, "_typed", False):
+        lib.hfg_default_options.argtypes = [C.POINTER(HfgOptions)]
+        lib.hfg_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.POINTER(HfgOptions)]
+        lib.hfg_set_basis.argtypes = [C.c_void_p, C.c_int, _ip, _ip, _ip, _dp, _dp, C.c_int, _ip, _ip]
+        lib.hfg_set_geometry.argtypes = [C.c_void_p, C.c_int, _dp, _ip]
+        lib.hfg_build_jk.argtypes = [C.c_void_p, _dp, _dp, _dp, C.c_int]
+        lib.hfg_fock.argtypes = [C.c_void_p, _dp, _dp, C.c_int, _dp]
+        lib.hfg_reset_incremental.argtypes = [C.c_void_p]
+        lib.hfg_energy.argtypes = [C.c_void_p, _dp, _dp]
+        lib.hfg_energy_backward.argtypes = [C.c_void_p, _dp, _dp]
+        lib.hfg_eri_shell_quartet.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, _dp]
+        lib.hfg_schwarz.argtypes = [C.c_void_p, _dp]
+        lib.hfg_last_stats.argtypes = [C.c_void_p, C.POINTER(HfgStats)]
+        lib.hfg_nbf.argtypes = [C.c_void_p]
+        lib.hfg_last_error.argtypes = [C.c_void_p]
+        lib.hfg_last_error.restype = C.c_char_p
+        lib.hfg_destroy.argtypes = [C.c_void_p]
+        lib._typed = True
+    return lib
+
+
+def hfscf():
+    """ctypes handle of libhfscf.so with argument types declared."""
+    lib = _load("libhfscf.so")
+    if not getattr(lib, "_typed", False):
+        lib.hfscf_default_scf_options.argtypes = [C.POINTER(ScfOptions)]
+        lib.hfscf_mol_create.argtypes = [C.c_int, _ip, _dp, C.c_char_p, C.c_int]
+        lib.hfscf_mol_create.restype = C.c_void_p
+        lib.hfscf_mol_from_xyz.argtypes = [C.c_char_p, C.c_char_p, C.c_int]
+        lib.hfscf_mol_from_xyz.restype = C.c_void_p
+        lib.hfscf_mol_destroy.argtypes = [C.c_void_p]
+        lib.hfscf_mol_info.argtypes = [C.c_void_p, _ip, _ip, _ip, _ip, _ip, _ip]
+        lib.hfscf_mol_nprim_total.argtypes = [C.c_void_p]
+        lib.hfscf_mol_shells.argtypes = [C.c_void_p, _ip, _ip, _ip, _ip, _ip, _dp, _dp, _dp]
+        lib.hfscf_mol_geometry.argtypes = [C.c_void_p, _ip, _dp]
+        lib.hfscf_mol_set_geometry.argtypes = [C.c_void_p, _dp]
+        lib.hfscf_mol_enuc.argtypes = [C.c_void_p]
+        lib.hfscf_mol_enuc.restype = C.c_double
+        lib.hfscf_oneint.argtypes = [C.c_void_p, _dp, _dp, _dp]
+        lib.hfscf_oneint_gradient.argtypes = [C.c_void_p, _dp, _dp, _dp]
+        lib.hfscf_rhf_run.argtypes = [C.c_void_p, C.POINTER(ScfOptions), C.POINTER(Callbacks),
+                                      C.POINTER(ScfResult), _dp, _dp, _dp]
+        lib.hfscf_rhf_gradient.argtypes = [C.c_void_p, C.POINTER(ScfOptions), C.POINTER(Callbacks), _dp, _dp, _dp]
+        lib.hfscf_last_error.restype = C.c_char_p
+        lib._typed = True
+    return lib
+
+
+def _ptr(a, dtype=np.float64):
+    return a.ctypes.data_as(_dp if dtype == np.float64 else _ip)
+
+
+def basis_path(name):
+    """hfscf's basis file naming: lowercase, '*' -> '_st_' (hfscf: src/molecule/hfscf_parser.hpp:134-151)."""
+    fn = name.lower().replace("*", "_st_") + ".gbs"
+    path = os.path.join(BASISDIR, fn)
+    if not os.path.exists(path):
+        raise FileNotFoundError(path)
+    return path
+
+
+class Molecule:
+    """A molecule with its basis (MOLEC::Molecule)."""
+
+    def __init__(self, handle):
+        if not handle:
+            raise RuntimeError(hfscf().hfscf_last_error().decode())
+        self._h = handle
+        lib = hfscf()
+        vals = [C.c_int() for _ in range(6)]
+        lib.hfscf_mol_info(self._h, *[C.byref(v) for v in vals])
+        self.natom, self.nshell, self.nbf, self.ncart, self.nelec, pure = [v.value for v in vals]
+        self.pure = bool(pure)
+
+    @classmethod
+    def from_xyz(cls, xyz_path, basis, pure=-1):
+        return cls(hfscf().hfscf_mol_from_xyz(str(xyz_path).encode(), basis_path(basis).encode(), pure))
+
+    @classmethod
+    def from_arrays(cls, Z, xyz_bohr, basis, pure=-1):
+        Z = np.ascontiguousarray(Z, dtype=np.int32)
+        x = np.ascontiguousarray(xyz_bohr, dtype=np.float64).reshape(-1)
+        return cls(hfscf().hfscf_mol_create(len(Z), Z.ctypes.data_as(_ip), _ptr(x), basis_path(basis).encode(), pure))
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            hfscf().hfscf_mol_destroy(self._h)
+            self._h = None
+
+    def shells(self):
+        """Shell table: dict of arrays (L, atom, nprim, cart_off, pure_off, alpha, coef, coef_raw)."""
+        lib = hfscf()
+        ns, npt = self.nshell, lib.hfscf_mol_nprim_total(self._h)
+        ints = [np.zeros(ns, dtype=np.int32) for _ in range(5)]
+        dbl = [np.zeros(npt) for _ in range(3)]
+        lib.hfscf_mol_shells(self._h, *[a.ctypes.data_as(_ip) for a in ints], *[_ptr(a) for a in dbl])
+        keys = ["L", "atom", "nprim", "cart_off", "pure_off"]
+        out = dict(zip(keys, ints))
+        out.update(alpha=dbl[0], coef=dbl[1], coef_raw=dbl[2])
+        return out
+
+    def geometry(self):
+        Z = np.zeros(self.natom, dtype=np.int32)
+        x = np.zeros(3 * self.natom)
+        hfscf().hfscf_mol_geometry(self._h, Z.ctypes.data_as(_ip), _ptr(x))
+        return Z, x.reshape(-1, 3)
+
+    def set_geometry(self, xyz_bohr):
+        x = np.ascontiguousarray(xyz_bohr, dtype=np.float64).reshape(-1)
+        hfscf().hfscf_mol_set_geometry(self._h, _ptr(x))
+
+    def enuc(self):
+        return hfscf().hfscf_mol_enuc(self._h)
+
+    def oneint(self):
+        n = self.nbf
+        S, T, V = np.zeros((n, n)), np.zeros((n, n)), np.zeros((n, n))
+        if hfscf().hfscf_oneint(self._h, _ptr(S), _ptr(T), _ptr(V)):
+            raise RuntimeError(hfscf().hfscf_last_error().decode())
+        return S, T, V
+
+    def oneint_gradient(self, D, Q):
+        g = np.zeros((self.natom, 3))
+        D = np.ascontiguousarray(D)
+        Q = np.ascontiguousarray(Q)
+        hfscf().hfscf_oneint_gradient(self._h, _ptr(D), _ptr(Q), _ptr(g))
+        return g
+
+
+def scf_options(**kw):
+    o = ScfOptions()
+    hfscf().hfscf_default_scf_options(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+def _callbacks(jk=None, grad2e=None, nbf=0, natom=0):
+    """Wrap Python callables as an hfscf_callbacks struct (keeps references alive on the struct)."""
+
+    def _jk(user, D, J, K):
+        d = np.ctypeslib.as_array(D, shape=(nbf, nbf))
+        j, k = jk(d.copy())
+        np.ctypeslib.as_array(J, shape=(nbf, nbf))[:] = j
+        np.ctypeslib.as_array(K, shape=(nbf, nbf))[:] = k
+        return 0
+
+    def _g(user, D, G):
+        d = np.ctypeslib.as_array(D, shape=(nbf, nbf))
+        np.ctypeslib.as_array(G, shape=(natom, 3))[:] = grad2e(d.copy())
+        return 0
+
+    cb = Callbacks()
+    cb._keep = (JKFUNC(_jk), GRADFUNC(_g))
+    cb.build_jk, cb.grad2e = cb._keep
+    return cb
+
+
+def rhf(mol, jk=None, grad2e=None, **opts):
+    """RHF SCF (CORE guess). With jk=None the GPU backend builds every Fock matrix; a Python
+    callable jk(D) -> (J, K) plugs into the same seam (used by tests to drive the host loop)."""
+    lib = hfscf()
+    o = scf_options(**opts)
+    cb = None
+    if jk is not None:
+        o.backend = 1
+        cb = _callbacks(jk, grad2e, mol.nbf, mol.natom)
+    n = mol.nbf
+    D, F = np.zeros((n, n)), np.zeros((n, n))
+    E = np.zeros(max(o.max_iter, 1) + 1)
+    res = ScfResult()
+    rc = lib.hfscf_rhf_run(mol._h, C.byref(o), C.byref(cb) if cb else None, C.byref(res), _ptr(D), _ptr(F), _ptr(E))
+    if rc not in (0, -2):
+        raise RuntimeError(lib.hfscf_last_error().decode())
+    return dict(energy=res.energy, energy_elec=res.energy_elec, energy_nuc=res.energy_nuc, iterations=res.iterations,
+                converged=bool(res.converged), fock_seconds=res.fock_seconds, D=D, F=F,
+                energies=E[:res.iterations + 1].copy())
+
+
+def rhf_gradient(mol, D, F, jk=None, grad2e=None, **opts):
+    """RHF analytic gradient (natom x 3, Eh/bohr)."""
+    lib = hfscf()
+    o = scf_options(**opts)
+    cb = None
+    if grad2e is not None:
+        o.backend = 1
+        cb = _callbacks(jk, grad2e, mol.nbf, mol.natom)
+    g = np.zeros((mol.natom, 3))
+    D = np.ascontiguousarray(D)
+    F = np.ascontiguousarray(F)
+    if lib.hfscf_rhf_gradient(mol._h, C.byref(o), C.byref(cb) if cb else None, _ptr(D), _ptr(F), _ptr(g)):
+        raise RuntimeError(lib.hfscf_last_error().decode())
+    return g
+
+
+class GpuContext:
+    """The GPU two-electron engine for one molecule (hfgpu C ABI)."""
+
+    def __init__(self, mol, precision=FP64, integral_tol=1e-14, schwarz=True, device=0):
+        lib = hfgpu()
+        opt = HfgOptions()
+        lib.hfg_default_options(C.byref(opt))
+        opt.precision, opt.integral_tol, opt.schwarz = precision, integral_tol, int(schwarz)
+        h = C.c_void_p()
+        rc = lib.hfg_create(C.byref(h), device, C.byref(opt))
+        if rc:
+            raise RuntimeError(f"hfg_create failed with status {rc} (no usable GPU?)")
+        self._h = h
+        self.mol = mol
+        self.nbf = mol.nbf
+        sh = mol.shells()
+        self._check(lib.hfg_set_basis(h, mol.nshell, sh["L"].ctypes.data_as(_ip), sh["atom"].ctypes.data_as(_ip),
+                                      sh["nprim"].ctypes.data_as(_ip), _ptr(sh["alpha"]), _ptr(sh["coef"]),
+                                      int(mol.pure), sh["cart_off"].ctypes.data_as(_ip),
+                                      sh["pure_off"].ctypes.data_as(_ip)))
+        self.set_geometry(mol.geometry()[1])
+
+    def _check(self, rc):
+        if rc:
+            raise RuntimeError(f"hfgpu status {rc}: {hfgpu().hfg_last_error(self._h).decode()}")
+
+    def set_geometry(self, xyz_bohr):
+        Z, _ = self.mol.geometry()
+        x = np.ascontiguousarray(xyz_bohr, dtype=np.float64).reshape(-1)
+        self._check(hfgpu().hfg_set_geometry(self._h, len(Z), _ptr(x), Z.ctypes.data_as(_ip)))
+
+    def build_jk(self, D):
+        n = self.nbf
+        D = np.ascontiguousarray(D, dtype=np.float64)
+        J, K = np.zeros((n, n)), np.zeros((n, n))
+        self._check(hfgpu().hfg_build_jk(self._h, _ptr(D), _ptr(J), _ptr(K), 0))
+        return J, K
+
+    def fock(self, H, D, incremental=False):
+        n = self.nbf
+        F = np.zeros((n, n))
+        H = np.ascontiguousarray(H, dtype=np.float64)
+        D = np.ascontiguousarray(D, dtype=np.float64)
+        self._check(hfgpu().hfg_fock(self._h, _ptr(H), _ptr(D), int(incremental), _ptr(F)))
+        return F
+
+    def energy(self, D):
+        e = C.c_double()
+        D = np.ascontiguousarray(D, dtype=np.float64)
+        self._check(hfgpu().hfg_energy(self._h, _ptr(D), C.byref(e)))
+        return e.value
+
+    def energy_backward(self, D):
+        g = np.zeros((self.mol.natom, 3))
+        D = np.ascontiguousarray(D, dtype=np.float64)
+        self._check(hfgpu().hfg_energy_backward(self._h, _ptr(D), _ptr(g)))
+        return g
+
+    def eri_block(self, s1, s2, s3, s4):
+        sh = self.mol.shells()
+        nf = [(2 * sh["L"][s] + 1) if self.mol.pure else (sh["L"][s] + 1) * (sh["L"][s] + 2) // 2
+              for s in (s1, s2, s3, s4)]
+        out = np.zeros(nf)
+        self._check(hfgpu().hfg_eri_shell_quartet(self._h, s1, s2, s3, s4, _ptr(out)))
+        return out
+
+    def schwarz(self):
+        S = np.zeros((self.mol.ncart, self.mol.ncart))
+        self._check(hfgpu().hfg_schwarz(self._h, _ptr(S)))
+        return S
+
+    def stats(self):
+        st = HfgStats()
+        hfgpu().hfg_last_stats(self._h, C.byref(st))
+        return {k: getattr(st, k) for k, _ in HfgStats._fields_}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            hfgpu().hfg_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()

@@ -1,0 +1,62 @@
+// Kernel declarations and per-class launch dispatch.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "layout.hpp"
+
+namespace hfg {
+
+struct JkArgs {
+    const WorkItem* items;
+    const PairDev* pairs;
+    const double* psmean;  // mean Schwarz value per pair (parallel to pairs)
+    const int* bp_off;     // block pair -> first pair
+    const int* bp_cnt;
+    const int* blk_nf;     // functions per block
+    const double4* pp0;
+    const double4* pp1;
+    const double4* pp2;
+    const double* boys;
+    const double* D;
+    int n;
+    double tol;        // shell-quartet Schwarz threshold on mean values
+    double prim_cut;   // primitive-quartet prefactor cutoff
+    double* slab;      // FP64 tiles per work item
+};
+
+struct GatherArgs {
+    const int2* tiles;     // (X, Y) per output tile, X >= Y
+    const int* gJ_off;     // CSR offsets per tile
+    const GatherEntry* gJ;
+    const int* gK_off;
+    const GatherEntry* gK;
+    const double* slab;
+    const int* blk_nf;
+    const int* blk_func_off;
+    const int* blk_func_ao;
+    int n;
+    double* J;
+    double* K;
+};
+
+struct ScreenArgs {  // Cartesian (ab|ab) diagonal per pair
+    const PairDev* pairs;
+    const int* cart_a;  // Cartesian offset of sa / sb per pair
+    const int* cart_b;
+    int count;
+    const double4* pp0;
+    const double4* pp1;
+    const double4* pp2;
+    const double* boys;
+    int ncart;
+    double* screen;  // ncart x ncart
+};
+
+// Launchers (one specialised kernel per class; la >= lb, la >= lc >= ld).
+hipError_t launch_jk(int la, int lb, int lc, int ld, bool pure, int mixed, const JkArgs& a, int nitems, hipStream_t s);
+hipError_t launch_screen(int la, int lb, const ScreenArgs& a, hipStream_t s);
+hipError_t launch_gather(const GatherArgs& a, int ntiles, hipStream_t s);
+hipError_t launch_fock(const double* base, const double* J, const double* K, double* F, double* Facc, int nn,
+                       int accumulate, hipStream_t s);
+
+}  // namespace hfg

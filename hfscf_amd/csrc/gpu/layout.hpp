@@ -1,0 +1,49 @@
+// Device data layout of the two-electron engine (shared by host set-up and kernels).
+//
+// Shells are grouped into "blocks": shells of one angular momentum, sorted by primitive count,
+// at most kBlockFuncs output functions per block. A block pair (X >= Y) owns the list of
+// significant shell pairs (a in X, b in Y), sorted by descending mean Schwarz value, and a
+// work item is a contiguous row range of one bra block pair against one ket block pair.
+// Every shell quartet of a work item adds into six small J/K tiles (blocks A x B, C x D,
+// A x C, A x D, B x C, B x D) held in LDS; the tiles are flushed to the work item's private
+// slab in HBM and summed per output tile by a gather kernel, so no global atomics are used.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace hfg {
+
+constexpr int kBlockFuncs = 16;   // output functions per shell block
+constexpr int kMaxRows = 256;     // bra rows per work item
+constexpr int kRegions = 6;       // Jab Jcd Kac Kad Kbc Kbd
+
+// One shell pair; primitive pairs live in pp0/pp1/pp2 at [off, off + np).
+struct PairDev {
+    int sa, sb;        // shell indices (input order)
+    int off, np;       // primitive-pair range
+    int slot_a, slot_b;  // positions of sa / sb inside their blocks
+    int ao_a, ao_b;    // first output-basis function of sa / sb
+    double AB[3];      // A - B
+};
+
+// Primitive-pair data (three double4 per primitive pair):
+//   pp0 = {p, Px, Py, Pz}
+//   pp1 = {PAx, PAy, PAz, K} with K = exp(-ab/p |AB|^2) c_a c_b sqrt(2 pi^2.5) / p
+//   pp2 = {1/(2p), alpha_a, alpha_b, 0}
+// so that the quartet prefactor is K_ab K_cd / sqrt(p + q).
+
+struct WorkItem {
+    int P, Q;          // bra / ket block pair
+    int r0, r1;        // bra rows [r0, r1) of P's pair list
+    int A, B, C, D;    // blocks
+    long long slab;    // offset (doubles) of this item's six tiles
+};
+
+// Gather list entry for one output tile: a tile of `rows x cols` at `off` (row-major),
+// read transposed when trans != 0.
+struct GatherEntry {
+    long long off;
+    int cols;
+    int trans;
+};
+
+}  // namespace hfg

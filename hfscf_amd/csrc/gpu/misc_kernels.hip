@@ -1,0 +1,99 @@
+// Schwarz diagonal, gather and Fock kernels.
+#include "jk_impl.hpp"
+
+namespace hfg {
+
+// ------------------------------------------------------------------ Schwarz diagonal (ab|ab)
+// hfscf keeps the Cartesian diagonal S_pq = (pq|pq) per shell pair (hfscf_oseri_screen.cpp:22-181).
+template <int LA, int LB>
+__global__ __launch_bounds__(64) void screen_kernel(ScreenArgs a) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.count) return;
+    constexpr int na = ncart(LA), nb = ncart(LB);
+    const PairDev p = a.pairs[t];
+    double blk[na * nb * na * nb];
+    eri_cart<LA, LB, LA, LB, double>(p, p, a.pp0, a.pp1, a.pp2, a.boys, 0.0, blk);
+    const int ca = a.cart_a[t], cb = a.cart_b[t];
+    for (int x = 0; x < na; ++x)
+        for (int y = 0; y < nb; ++y) {
+            const double v = blk[((x * nb + y) * na + x) * nb + y];
+            a.screen[(size_t)(ca + x) * a.ncart + cb + y] = v;
+            a.screen[(size_t)(cb + y) * a.ncart + ca + x] = v;
+        }
+}
+
+hipError_t launch_screen(int la, int lb, const ScreenArgs& a, hipStream_t s) {
+    if (a.count == 0) return hipSuccess;
+    const dim3 grid((a.count + 63) / 64), block(64);
+#define HFG_SCREEN(A, B)                                                   \
+    if (la == A && lb == B) {                                              \
+        hipLaunchKernelGGL((screen_kernel<A, B>), grid, block, 0, s, a);   \
+        return hipGetLastError();                                          \
+    }
+    HFG_SCREEN(0, 0)
+    HFG_SCREEN(1, 0)
+    HFG_SCREEN(1, 1)
+    HFG_SCREEN(2, 0)
+    HFG_SCREEN(2, 1)
+    HFG_SCREEN(2, 2)
+#undef HFG_SCREEN
+    return hipErrorInvalidValue;
+}
+
+// ------------------------------------------------------------------ gather
+// One workgroup per output tile (X >= Y): sums every work-item tile that targets it
+// (transposed where the work item held it as (Y, X)) in a fixed order, then writes the
+// symmetric J and K elements.
+__global__ __launch_bounds__(256) void gather_kernel(GatherArgs a) {
+    const int2 xy = a.tiles[blockIdx.x];
+    const int fX = a.blk_nf[xy.x], fY = a.blk_nf[xy.y];
+    const int* aoX = a.blk_func_ao + a.blk_func_off[xy.x];
+    const int* aoY = a.blk_func_ao + a.blk_func_off[xy.y];
+    const int j0 = a.gJ_off[blockIdx.x], j1 = a.gJ_off[blockIdx.x + 1];
+    const int k0 = a.gK_off[blockIdx.x], k1 = a.gK_off[blockIdx.x + 1];
+    for (int e = threadIdx.x; e < fX * fY; e += blockDim.x) {
+        const int u = e / fY, v = e - u * fY;
+        if (xy.x == xy.y && u < v) continue;  // diagonal tile: one thread per symmetric element pair
+        double sj = 0.0, sk = 0.0;
+        for (int g = j0; g < j1; ++g) {
+            const GatherEntry en = a.gJ[g];
+            sj += en.trans ? a.slab[en.off + (long long)v * en.cols + u] : a.slab[en.off + (long long)u * en.cols + v];
+        }
+        for (int g = k0; g < k1; ++g) {
+            const GatherEntry en = a.gK[g];
+            sk += en.trans ? a.slab[en.off + (long long)v * en.cols + u] : a.slab[en.off + (long long)u * en.cols + v];
+        }
+        const size_t i1 = (size_t)aoX[u] * a.n + aoY[v];
+        const size_t i2 = (size_t)aoY[v] * a.n + aoX[u];
+        a.J[i1] = sj;
+        a.J[i2] = sj;
+        a.K[i1] = sk;
+        a.K[i2] = sk;
+    }
+}
+
+hipError_t launch_gather(const GatherArgs& a, int ntiles, hipStream_t s) {
+    if (ntiles == 0) return hipSuccess;
+    hipLaunchKernelGGL(gather_kernel, dim3(ntiles), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ Fock
+// G = 2J - K (+ Facc when accumulating); F = base + G; Facc keeps G for incremental builds.
+__global__ void fock_kernel(const double* base, const double* J, const double* K, double* F, double* Facc, int nn,
+                            int accumulate) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nn) return;
+    double g = 2.0 * J[i] - K[i];
+    if (accumulate) g += Facc[i];
+    Facc[i] = g;
+    F[i] = base[i] + g;
+}
+
+hipError_t launch_fock(const double* base, const double* J, const double* K, double* F, double* Facc, int nn,
+                       int accumulate, hipStream_t s) {
+    hipLaunchKernelGGL(fock_kernel, dim3((nn + 255) / 256), dim3(256), 0, s, base, J, K, F, Facc, nn, accumulate);
+    return hipGetLastError();
+}
+
+}  // namespace hfg

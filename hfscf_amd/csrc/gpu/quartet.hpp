@@ -1,0 +1,140 @@
+// Contracted shell-quartet evaluation and J/K digestion for one (la lb|lc ld) class.
+#pragma once
+#include "boys.hpp"
+#include "hgp.hpp"
+#include "layout.hpp"
+
+namespace hfg {
+
+// Contracted Cartesian block (ab|cd) of one shell quartet, out[a][b][c][d] (ncart each).
+// prim_cut mirrors hfscf's primitive cutoff |pfac| < integral_tol/1e4
+// (hfscf: src/integrals/hfscf_oseri.cpp:73,98-101).
+template <int LA, int LB, int LC, int LD, typename R>
+HFG_INLINE void eri_cart(const PairDev& bp, const PairDev& kp, const double4* __restrict__ pp0,
+                         const double4* __restrict__ pp1, const double4* __restrict__ pp2,
+                         const double* __restrict__ boys_tab, double prim_cut, R* out) {
+    constexpr int LT = LA + LB + LC + LD;
+    using H = Hrr<LA, LB, LC, LD, R>;
+    R acc[H::NE][H::NF];
+    sfor<H::NE>([&](auto e) { sfor<H::NF>([&](auto f) { acc[e][f] = (R)0; }); });
+    for (int i = 0; i < bp.np; ++i) {
+        const double4 a0 = pp0[bp.off + i];
+        const double4 a1 = pp1[bp.off + i];
+        const double o2p = pp2[bp.off + i].x;
+        const double p = a0.x;
+        for (int j = 0; j < kp.np; ++j) {
+            const double4 b0 = pp0[kp.off + j];
+            const double4 b1 = pp1[kp.off + j];
+            const double q = b0.x;
+            const double rs = rsqrt(p + q);
+            const double pfac = a1.w * b1.w * rs;
+            if (fabs(pfac) < prim_cut) continue;
+            const double PQx = a0.y - b0.y, PQy = a0.z - b0.z, PQz = a0.w - b0.w;
+            const double r2 = PQx * PQx + PQy * PQy + PQz * PQz;
+            const double ipq = rs * rs;
+            const double rho = p * q * ipq;
+            double T = rho * r2;
+            R F[LT + 1];
+            if constexpr (LT == 0) {
+                // hfscf floors T at 1e-8 inside F_nu and uses F_n = 1/(2n+1) when r_PQ^2 <= 1e-18
+                T = (r2 > 1e-18) ? fmax(T, 1e-8) : 0.0;
+                boys_eval<0, R>(T, boys_tab, pfac, F);
+                acc[0][0] += F[0];
+            } else {
+                boys_eval<LT, R>(T, boys_tab, pfac, F);
+                const double rp = q * ipq, rq = p * ipq;
+                const R PA[3] = {(R)a1.x, (R)a1.y, (R)a1.z};
+                const R QC[3] = {(R)b1.x, (R)b1.y, (R)b1.z};
+                const R WP[3] = {(R)(-rp * PQx), (R)(-rp * PQy), (R)(-rp * PQz)};
+                const R WQ[3] = {(R)(rq * PQx), (R)(rq * PQy), (R)(rq * PQz)};
+                Vrr<LA + LB, LC + LD, R> V;
+                V.run(F, PA, WP, QC, WQ, (R)o2p, (R)rp, (R)pp2[kp.off + j].x, (R)rq, (R)(0.5 * ipq));
+                sfor<H::NE>([&](auto e) {
+                    sfor<H::NF>([&](auto f) { acc[e][f] += V.v[H::NE0 + e][H::NF0 + f][0]; });
+                });
+            }
+        }
+    }
+    const R AB[3] = {(R)bp.AB[0], (R)bp.AB[1], (R)bp.AB[2]};
+    const R CD[3] = {(R)kp.AB[0], (R)kp.AB[1], (R)kp.AB[2]};
+    H::run(acc, AB, CD, out);
+}
+
+// Output-basis block of one quartet (spherical when PURE), out[a][b][c][d].
+template <int LA, int LB, int LC, int LD, bool PURE, typename R>
+HFG_INLINE void eri_block(const PairDev& bp, const PairDev& kp, const double4* __restrict__ pp0,
+                          const double4* __restrict__ pp1, const double4* __restrict__ pp2,
+                          const double* __restrict__ boys_tab, double prim_cut, R* out) {
+    using S = Sph<LA, LB, LC, LD, PURE, R>;
+    if constexpr (S::any) {
+        R cart[S::c0 * S::c1 * S::c2 * S::c3];
+        eri_cart<LA, LB, LC, LD, R>(bp, kp, pp0, pp1, pp2, boys_tab, prim_cut, cart);
+        S::run(cart, out);
+    } else {
+        eri_cart<LA, LB, LC, LD, R>(bp, kp, pp0, pp1, pp2, boys_tab, prim_cut, out);
+    }
+}
+
+// Adds one quartet's contributions into the six LDS tiles. I is the output-basis block,
+// s the permutational weight (1/2 for each of a==b, c==d, ab==cd). J tiles receive
+// 2 s D I and K tiles s D I; the full matrices are J = T + T^T and K = T + T^T.
+template <int NA, int NB, int NC, int ND, typename R, typename ACC>
+HFG_INLINE void digest(const R* I, double s, const double* __restrict__ D, int n, const PairDev& bp,
+                       const PairDev& kp, ACC* tJab, ACC* tJcd, ACC* tKac, ACC* tKad, ACC* tKbc, ACC* tKbd, int fB,
+                       int fC, int fD) {
+    const int ao_a = bp.ao_a, ao_b = bp.ao_b, ao_c = kp.ao_a, ao_d = kp.ao_b;
+    const int la0 = bp.slot_a * NA, lb0 = bp.slot_b * NB, lc0 = kp.slot_a * NC, ld0 = kp.slot_b * ND;
+    R dab[NA][NB], dcd[NC][ND], dac[NA][NC], dad[NA][ND], dbc[NB][NC], dbd[NB][ND];
+    sfor<NA>([&](auto p) {
+        sfor<NB>([&](auto q) { dab[p][q] = (R)D[(size_t)(ao_a + p) * n + ao_b + q]; });
+        sfor<NC>([&](auto r) { dac[p][r] = (R)D[(size_t)(ao_a + p) * n + ao_c + r]; });
+        sfor<ND>([&](auto t) { dad[p][t] = (R)D[(size_t)(ao_a + p) * n + ao_d + t]; });
+    });
+    sfor<NB>([&](auto q) {
+        sfor<NC>([&](auto r) { dbc[q][r] = (R)D[(size_t)(ao_b + q) * n + ao_c + r]; });
+        sfor<ND>([&](auto t) { dbd[q][t] = (R)D[(size_t)(ao_b + q) * n + ao_d + t]; });
+    });
+    sfor<NC>([&](auto r) { sfor<ND>([&](auto t) { dcd[r][t] = (R)D[(size_t)(ao_c + r) * n + ao_d + t]; }); });
+    const R sj = (R)(2.0 * s), sk = (R)s;
+    // J_ab and K_ac / K_ad
+    sfor<NA>([&](auto p) {
+        sfor<NB>([&](auto q) {
+            R j = 0;
+            sfor<NC>([&](auto r) { sfor<ND>([&](auto t) { j += dcd[r][t] * I[((p * NB + q) * NC + r) * ND + t]; }); });
+            atomicAdd(&tJab[(la0 + p) * fB + lb0 + q], (ACC)(sj * j));
+        });
+        sfor<NC>([&](auto r) {
+            R k = 0;
+            sfor<NB>([&](auto q) { sfor<ND>([&](auto t) { k += dbd[q][t] * I[((p * NB + q) * NC + r) * ND + t]; }); });
+            atomicAdd(&tKac[(la0 + p) * fC + lc0 + r], (ACC)(sk * k));
+        });
+        sfor<ND>([&](auto t) {
+            R k = 0;
+            sfor<NB>([&](auto q) { sfor<NC>([&](auto r) { k += dbc[q][r] * I[((p * NB + q) * NC + r) * ND + t]; }); });
+            atomicAdd(&tKad[(la0 + p) * fD + ld0 + t], (ACC)(sk * k));
+        });
+    });
+    // J_cd
+    sfor<NC>([&](auto r) {
+        sfor<ND>([&](auto t) {
+            R j = 0;
+            sfor<NA>([&](auto p) { sfor<NB>([&](auto q) { j += dab[p][q] * I[((p * NB + q) * NC + r) * ND + t]; }); });
+            atomicAdd(&tJcd[(lc0 + r) * fD + ld0 + t], (ACC)(sj * j));
+        });
+    });
+    // K_bc / K_bd
+    sfor<NB>([&](auto q) {
+        sfor<NC>([&](auto r) {
+            R k = 0;
+            sfor<NA>([&](auto p) { sfor<ND>([&](auto t) { k += dad[p][t] * I[((p * NB + q) * NC + r) * ND + t]; }); });
+            atomicAdd(&tKbc[(lb0 + q) * fC + lc0 + r], (ACC)(sk * k));
+        });
+        sfor<ND>([&](auto t) {
+            R k = 0;
+            sfor<NA>([&](auto p) { sfor<NC>([&](auto r) { k += dac[p][r] * I[((p * NB + q) * NC + r) * ND + t]; }); });
+            atomicAdd(&tKbd[(lb0 + q) * fD + ld0 + t], (ACC)(sk * k));
+        });
+    });
+}
+
+}  // namespace hfg

@@ -1,0 +1,302 @@
+// C ABI of the host RHF layer and the GPU two-electron backend (GpuJK) that plugs the
+// hfgpu library into the hf_solver seam.
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+
+#include "../../../include/hfgpu.h"
+#include "../../../include/hfscf.h"
+#include "basis.hpp"
+#include "linalg.hpp"
+#include "oneint.hpp"
+#include "scf.hpp"
+
+using namespace hfscf;
+
+struct hfscf_mol {
+    Molecule mol;
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+int set_err(const std::string& s) {
+    g_err = s;
+    return -1;
+}
+
+// GPU backend: ERIOS/hf_solver/scf_gradient two-electron work through the hfgpu C ABI.
+class GpuJK final : public JKBuilder {
+   public:
+    GpuJK(const Molecule& mol, const hfscf_scf_options& o) : mol_(mol) {
+        hfg_options go;
+        hfg_default_options(&go);
+        go.precision = o.precision;
+        go.integral_tol = o.integral_tol;
+        go.schwarz = o.schwarz;
+        int r = hfg_create(&ctx_, o.device, &go);
+        if (r) throw std::runtime_error("hfg_create failed (" + std::to_string(r) + "): no usable GPU");
+        const int ns = (int)mol.shells.size();
+        std::vector<int> L(ns), atom(ns), np(ns), co(ns), po(ns);
+        std::vector<double> a, c;
+        for (int s = 0; s < ns; ++s) {
+            const Shell& sh = mol.shells[s];
+            L[s] = sh.L;
+            atom[s] = sh.atom;
+            np[s] = (int)sh.alpha.size();
+            co[s] = sh.idx;
+            po[s] = sh.ids;
+            a.insert(a.end(), sh.alpha.begin(), sh.alpha.end());
+            c.insert(c.end(), sh.c.begin(), sh.c.end());
+        }
+        check(hfg_set_basis(ctx_, ns, L.data(), atom.data(), np.data(), a.data(), c.data(), mol.pure ? 1 : 0,
+                            co.data(), po.data()));
+        std::vector<double> xyz;
+        for (auto& p : mol.xyz) xyz.insert(xyz.end(), p.begin(), p.end());
+        check(hfg_set_geometry(ctx_, mol.natom(), xyz.data(), mol.Z.data()));
+    }
+    ~GpuJK() override { hfg_destroy(ctx_); }
+    void build_jk(const double* D, double* J, double* K) override { check(hfg_build_jk(ctx_, D, J, K, 0)); }
+    void fock(const double* H, const double* D, int incremental, double* F) override {
+        check(hfg_fock(ctx_, H, D, incremental, F));
+    }
+    void gradient_2e(const double* D, double* grad) override {
+        std::vector<double> g((size_t)mol_.natom() * 3);
+        check(hfg_energy_backward(ctx_, D, g.data()));
+        for (size_t k = 0; k < g.size(); ++k) grad[k] += g[k];
+    }
+
+   private:
+    void check(int r) {
+        if (r) throw std::runtime_error(std::string("hfgpu: ") + hfg_last_error(ctx_));
+    }
+    const Molecule& mol_;
+    hfg_ctx* ctx_ = nullptr;
+};
+
+// Callback backend (an external Fock builder behind the same seam).
+class CallbackJK final : public JKBuilder {
+   public:
+    CallbackJK(const Molecule& mol, const hfscf_callbacks& cb) : mol_(mol), cb_(cb) {}
+    void build_jk(const double* D, double* J, double* K) override {
+        if (!cb_.build_jk || cb_.build_jk(cb_.user, D, J, K)) throw std::runtime_error("callback build_jk failed");
+    }
+    void fock(const double* H, const double* D, int incremental, double* F) override {
+        (void)incremental;
+        const size_t nn = (size_t)mol_.nbf() * mol_.nbf();
+        std::vector<double> J(nn), K(nn);
+        build_jk(D, J.data(), K.data());
+        for (size_t k = 0; k < nn; ++k) F[k] = H[k] + 2.0 * J[k] - K[k];
+    }
+    void gradient_2e(const double* D, double* grad) override {
+        std::vector<double> g((size_t)mol_.natom() * 3);
+        if (!cb_.grad2e || cb_.grad2e(cb_.user, D, g.data())) throw std::runtime_error("callback grad2e failed");
+        for (size_t k = 0; k < g.size(); ++k) grad[k] += g[k];
+    }
+
+   private:
+    const Molecule& mol_;
+    hfscf_callbacks cb_;
+};
+
+std::unique_ptr<JKBuilder> make_backend(const Molecule& mol, const hfscf_scf_options& o, const hfscf_callbacks* cb) {
+    if (o.backend == HFSCF_BACKEND_CALLBACK) {
+        if (!cb) throw std::runtime_error("callback backend selected without callbacks");
+        return std::make_unique<CallbackJK>(mol, *cb);
+    }
+    if (o.backend != HFSCF_BACKEND_GPU) throw std::runtime_error("unknown backend");
+    return std::make_unique<GpuJK>(mol, o);
+}
+
+}  // namespace
+
+extern "C" {
+
+void hfscf_default_scf_options(hfscf_scf_options* o) {
+    std::memset(o, 0, sizeof(*o));
+    o->backend = HFSCF_BACKEND_GPU;
+    o->device = 0;
+    o->precision = 0;
+    o->incremental = 0;
+    o->diis_range = 6;
+    o->max_iter = 100;
+    o->schwarz = 1;
+    o->energy_tol = 1e-10;
+    o->rms_tol = 1e-10;
+    o->integral_tol = 1e-14;
+    o->final_fp64 = 1;
+}
+
+hfscf_mol* hfscf_mol_create(int natom, const int* Z, const double* xyz_bohr, const char* gbs_path, int pure_override) {
+    try {
+        std::vector<int> z(Z, Z + natom);
+        std::vector<std::array<double, 3>> x(natom);
+        for (int a = 0; a < natom; ++a) x[a] = {xyz_bohr[3 * a], xyz_bohr[3 * a + 1], xyz_bohr[3 * a + 2]};
+        auto* m = new hfscf_mol();
+        m->mol = build_molecule(z, x, read_gbs(gbs_path), pure_override);
+        return m;
+    } catch (const std::exception& e) {
+        set_err(e.what());
+        return nullptr;
+    }
+}
+
+hfscf_mol* hfscf_mol_from_xyz(const char* xyz_path, const char* gbs_path, int pure_override) {
+    try {
+        std::vector<int> z;
+        std::vector<std::array<double, 3>> x;
+        read_xyz(xyz_path, z, x);
+        auto* m = new hfscf_mol();
+        m->mol = build_molecule(z, x, read_gbs(gbs_path), pure_override);
+        return m;
+    } catch (const std::exception& e) {
+        set_err(e.what());
+        return nullptr;
+    }
+}
+
+void hfscf_mol_destroy(hfscf_mol* m) { delete m; }
+
+int hfscf_mol_info(const hfscf_mol* m, int* natom, int* nshell, int* nbf, int* ncart, int* nelec, int* pure) {
+    if (!m) return set_err("null molecule");
+    if (natom) *natom = m->mol.natom();
+    if (nshell) *nshell = (int)m->mol.shells.size();
+    if (nbf) *nbf = m->mol.nbf();
+    if (ncart) *ncart = m->mol.ncart_total;
+    if (nelec) *nelec = m->mol.nelectrons();
+    if (pure) *pure = m->mol.pure ? 1 : 0;
+    return 0;
+}
+
+int hfscf_mol_nprim_total(const hfscf_mol* m) {
+    int n = 0;
+    for (auto& s : m->mol.shells) n += (int)s.alpha.size();
+    return n;
+}
+
+int hfscf_mol_shells(const hfscf_mol* m, int* L, int* atom, int* nprim, int* cart_off, int* pure_off, double* alpha,
+                     double* coef_norm, double* coef_raw) {
+    if (!m) return set_err("null molecule");
+    int k = 0;
+    for (size_t s = 0; s < m->mol.shells.size(); ++s) {
+        const Shell& sh = m->mol.shells[s];
+        L[s] = sh.L;
+        atom[s] = sh.atom;
+        nprim[s] = (int)sh.alpha.size();
+        cart_off[s] = sh.idx;
+        pure_off[s] = sh.ids;
+        for (size_t p = 0; p < sh.alpha.size(); ++p, ++k) {
+            alpha[k] = sh.alpha[p];
+            coef_norm[k] = sh.c[p];
+            coef_raw[k] = sh.c_raw[p];
+        }
+    }
+    return 0;
+}
+
+int hfscf_mol_geometry(const hfscf_mol* m, int* Z, double* xyz_bohr) {
+    if (!m) return set_err("null molecule");
+    for (int a = 0; a < m->mol.natom(); ++a) {
+        if (Z) Z[a] = m->mol.Z[a];
+        for (int d = 0; d < 3; ++d) xyz_bohr[3 * a + d] = m->mol.xyz[a][d];
+    }
+    return 0;
+}
+
+int hfscf_mol_set_geometry(hfscf_mol* m, const double* xyz_bohr) {
+    if (!m) return set_err("null molecule");
+    std::vector<std::array<double, 3>> x(m->mol.natom());
+    for (int a = 0; a < m->mol.natom(); ++a) x[a] = {xyz_bohr[3 * a], xyz_bohr[3 * a + 1], xyz_bohr[3 * a + 2]};
+    m->mol.set_geometry(x);
+    return 0;
+}
+
+double hfscf_mol_enuc(const hfscf_mol* m) { return m ? m->mol.nuclear_repulsion() : 0.0; }
+
+int hfscf_oneint(const hfscf_mol* m, double* S, double* T, double* V) {
+    if (!m) return set_err("null molecule");
+    try {
+        std::vector<double> s, t, v;
+        overlap_kinetic(m->mol, s, t);
+        nuclear_attraction(m->mol, v);
+        if (S) std::memcpy(S, s.data(), s.size() * sizeof(double));
+        if (T) std::memcpy(T, t.data(), t.size() * sizeof(double));
+        if (V) std::memcpy(V, v.data(), v.size() * sizeof(double));
+        return 0;
+    } catch (const std::exception& e) {
+        return set_err(e.what());
+    }
+}
+
+int hfscf_oneint_gradient(const hfscf_mol* m, const double* D, const double* Q, double* grad) {
+    if (!m) return set_err("null molecule");
+    std::memset(grad, 0, sizeof(double) * 3 * m->mol.natom());
+    oneint_gradient(m->mol, D, Q, grad);
+    nuclear_repulsion_gradient(m->mol, grad);
+    return 0;
+}
+
+int hfscf_rhf_run(hfscf_mol* m, const hfscf_scf_options* o, const hfscf_callbacks* cb, hfscf_scf_result* res,
+                  double* D, double* F, double* energies) {
+    if (!m) return set_err("null molecule");
+    try {
+        hfscf_scf_options opt;
+        if (o)
+            opt = *o;
+        else
+            hfscf_default_scf_options(&opt);
+        auto jk = make_backend(m->mol, opt, cb);
+        ScfOptions so;
+        so.diis_range = opt.diis_range;
+        so.energy_tol = opt.energy_tol;
+        so.rms_tol = opt.rms_tol;
+        so.max_iter = opt.max_iter;
+        so.incremental = opt.incremental != 0;
+        so.verbose = opt.verbose;
+        hf_solver solver(m->mol, so, jk.get());
+        solver.init_data();
+        bool conv = solver.scf_run();
+        if (res) {
+            res->energy = solver.energy_total();
+            res->energy_elec = solver.energy_electronic();
+            res->energy_nuc = m->mol.nuclear_repulsion();
+            res->iterations = solver.iterations();
+            res->converged = conv ? 1 : 0;
+            res->fock_seconds = solver.total_fock_seconds();
+            res->last_fock_seconds = solver.last_fock_seconds();
+            res->fock_builds = solver.iterations();
+        }
+        const size_t nn = (size_t)m->mol.nbf() * m->mol.nbf();
+        if (D) std::memcpy(D, solver.density().data(), nn * sizeof(double));
+        if (F) std::memcpy(F, solver.fock().data(), nn * sizeof(double));
+        if (energies)
+            for (size_t k = 0; k < solver.energies().size(); ++k) energies[k] = solver.energies()[k];
+        return conv ? 0 : -2;
+    } catch (const std::exception& e) {
+        return set_err(e.what());
+    }
+}
+
+int hfscf_rhf_gradient(hfscf_mol* m, const hfscf_scf_options* o, const hfscf_callbacks* cb, const double* D,
+                       const double* F, double* grad) {
+    if (!m) return set_err("null molecule");
+    try {
+        hfscf_scf_options opt;
+        if (o)
+            opt = *o;
+        else
+            hfscf_default_scf_options(&opt);
+        auto jk = make_backend(m->mol, opt, cb);
+        auto g = calc_scf_gradient_rhf(m->mol, D, F, jk.get());
+        std::memcpy(grad, g.data(), g.size() * sizeof(double));
+        return 0;
+    } catch (const std::exception& e) {
+        return set_err(e.what());
+    }
+}
+
+const char* hfscf_last_error(void) { return g_err.c_str(); }
+
+}  // extern "C"

@@ -1,0 +1,95 @@
+// Restricted Hartree-Fock SCF driver and RHF gradient, shaped like hfscf's
+// Mol::hf_solver (hfscf: src/scfsolver/hfscf_hf_solver.hpp:23-121),
+// DIIS_SOLVER::diis_solver (hfscf: src/scfsolver/hfscf_diis.hpp:18-63) and
+// Mol::scf_gradient::calc_scf_gradient_rhf (hfscf: src/gradient/hfscf_gradient.hpp:34-37).
+//
+// The two-electron work (J/K Fock build and the two-electron gradient contraction) goes
+// through a JKBuilder, the run-time-selectable seam that the GPU library implements.
+#pragma once
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "basis.hpp"
+
+namespace hfscf {
+
+// Two-electron backend. D is the half density C_occ C_occ^T (nbf x nbf, row-major).
+struct JKBuilder {
+    virtual ~JKBuilder() = default;
+    // J_ij = sum_kl D_kl (ij|kl), K_ij = sum_kl D_kl (ik|jl)
+    virtual void build_jk(const double* D, double* J, double* K) = 0;
+    // F = H + 2J - K. With incremental != 0 the backend may build from D - D_prev and add
+    // to its own previous Fock matrix.
+    virtual void fock(const double* H, const double* D, int incremental, double* F) = 0;
+    // Two-electron part of the RHF gradient, sum_ijkl D_ij D_kl (2 (ij|kl)' - (ik|jl)'), natom x 3.
+    virtual void gradient_2e(const double* D, double* grad) = 0;
+    // Full-precision rebuild hint for the final iteration of a mixed-precision run.
+    virtual void set_final(bool final_build) { (void)final_build; }
+};
+
+struct ScfOptions {
+    int diis_range = 6;  // hfscf keeps diis_range+1 vectors (hfscf: src/scfsolver/hfscf_diis.cpp:29-47)
+    double energy_tol = 1e-10;
+    double rms_tol = 1e-10;
+    int max_iter = 100;
+    int charge = 0;
+    bool incremental = false;  // incremental (delta-D) Fock builds
+    int verbose = 0;
+};
+
+class diis_solver {
+   public:
+    explicit diis_solver(int range) : range_(range) {}
+    // hfscf: src/scfsolver/hfscf_diis.cpp:9-123
+    void diis_extrapolate(int n, double* F, const double* S, const double* D, const double* X);
+    double current_rms() const { return rms_; }
+
+   private:
+    int range_;
+    double rms_ = 1.0;
+    std::vector<std::vector<double>> e_list_, f_list_;
+    std::vector<double> rms_list_;
+};
+
+class hf_solver {
+   public:
+    hf_solver(const Molecule& mol, const ScfOptions& opt, JKBuilder* jk);
+    virtual ~hf_solver() = default;
+    void init_data();  // hfscf: src/scfsolver/hfscf_hf_solver.cpp:79-182 (CORE guess)
+    bool scf_run();    // hfscf: src/scfsolver/hfscf_hf_solver.cpp:389-476
+
+    double energy_electronic() const { return e_cur_; }
+    double energy_total() const { return e_cur_ + mol_.nuclear_repulsion(); }
+    int iterations() const { return iteration_; }
+    int nbf() const { return n_; }
+    const std::vector<double>& density() const { return D_; }
+    const std::vector<double>& fock() const { return F_; }
+    const std::vector<double>& hcore() const { return H_; }
+    const std::vector<double>& overlap() const { return S_; }
+    const std::vector<double>& mo_energies() const { return eps_; }
+    const std::vector<double>& energies() const { return energies_; }
+    double last_fock_seconds() const { return fock_seconds_; }
+    double total_fock_seconds() const { return fock_total_; }
+
+   protected:
+    virtual void update_fock_matrix();  // hfscf: src/scfsolver/hfscf_hf_solver.cpp:254-286
+    void calc_density_matrix();         // hfscf: src/scfsolver/hfscf_hf_solver.cpp:206-242
+    void calc_scf_energy();             // hfscf: src/scfsolver/hfscf_hf_solver.cpp:244-252
+
+    const Molecule& mol_;
+    ScfOptions opt_;
+    JKBuilder* jk_;
+    int n_ = 0, nocc_ = 0, iteration_ = 0;
+    std::vector<double> S_, T_, V_, H_, X_, F_, Fp_, D_, C_, eps_, energies_;
+    double e_cur_ = 0.0, e_prev_ = 0.0;
+    double fock_seconds_ = 0.0, fock_total_ = 0.0;
+    std::unique_ptr<diis_solver> diis_;
+};
+
+// RHF analytic gradient (natom x 3) from the converged half density D and Fock matrix F:
+// g = sum D.dG/dX + 2 sum D.(dT + dV) - 2 sum dS.(D F D) + dV_NN
+// (hfscf: src/gradient/hfscf_grad_rhf.cpp:8-178).
+std::vector<double> calc_scf_gradient_rhf(const Molecule& mol, const double* D, const double* F, JKBuilder* jk);
+
+}  // namespace hfscf

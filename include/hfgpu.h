@@ -1,0 +1,120 @@
+/*
+ * hfgpu — C ABI of the MI355X (gfx950) RHF two-electron engine.
+ *
+ * The library replaces hfscf's two-electron hot path behind three C++ seams:
+ *   - ERIOS::Erios, the shell-quartet ERI engine with Schwarz screening
+ *     (hfscf: src/integrals/hfscf_oseri.hpp:27-123, hfscf_oseri.cpp:22-246,
+ *      hfscf_oseri_screen.cpp:17-181),
+ *   - Mol::hf_solver::update_fock_matrix / update_fock_matrix_scf_direct, the J/K Fock build
+ *     (hfscf: src/scfsolver/hfscf_hf_solver.hpp:105-107, hfscf_hf_solver.cpp:254-373),
+ *   - the two-electron part of Mol::scf_gradient::calc_scf_gradient_rhf
+ *     (hfscf: src/gradient/hfscf_grad_rhf.cpp:76-141, hfscf_grad_ints.cpp:79-151).
+ *
+ * Conventions
+ *   - All host pointers are caller-owned and row-major; calls are synchronous on return.
+ *   - D is hfscf's half density C_occ C_occ^T; F = H + sum_kl D_kl (2 (ij|kl) - (ik|jl)),
+ *     J_ij = sum_kl D_kl (ij|kl), K_ij = sum_kl D_kl (ik|jl) (hfscf_hf_solver.cpp:221-231,261-278).
+ *   - Matrices use the output basis: spherical (m = -l..l) when pure != 0, else Cartesian,
+ *     functions ordered as hfscf orders them (shell order, Cartesian lx-major order).
+ *   - Status 0 = success, negative = error; hfg_last_error() gives the message. The library
+ *     never exits the process and never falls back to a CPU path.
+ *   - A context owns one HIP stream and all device memory; it is not thread-safe.
+ */
+#ifndef HFGPU_H
+#define HFGPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct hfg_ctx hfg_ctx;
+
+enum hfg_status {
+    HFG_OK = 0,
+    HFG_ERR_ARG = -1,      /* invalid argument or call order */
+    HFG_ERR_HIP = -2,      /* HIP runtime error (message in hfg_last_error) */
+    HFG_ERR_NOMEM = -3,    /* device allocation failed */
+    HFG_ERR_UNSUPPORTED = -4 /* angular momentum above d, or no GPU */
+};
+
+enum hfg_precision {
+    HFG_FP64 = 0,  /* every ERI and digestion in FP64 */
+    HFG_MIXED = 1  /* FP32 recursions/digestion, FP64 Boys, Schwarz, tile sums and Fock sum */
+};
+
+typedef struct hfg_options {
+    int precision;        /* hfg_precision */
+    double integral_tol;  /* hfscf integral_tol (default 1e-14): Schwarz threshold is 100x this,
+                             primitive cutoff integral_tol/1e4 (hfscf_oseri.hpp:31-32, hfscf_oseri.cpp:73) */
+    int schwarz;          /* 1 = screen shell quartets on the mean Schwarz bound (hfscf eri_screen) */
+    int reserved[8];
+} hfg_options;
+
+/* Fill opt with defaults (FP64, integral_tol 1e-14, screening on). */
+void hfg_default_options(hfg_options* opt);
+
+/* Create a context on HIP device `device`. Replaces: ERIOS::Erios::Erios(is_pure, cut_off). */
+int hfg_create(hfg_ctx** out, int device, const hfg_options* opt);
+
+/* Basis description, shells in hfscf order (hfscf: src/basis/hfscf_basis.cpp:245-275).
+ * L[s] <= 2; nprim[s] primitives; alpha / coef concatenated over shells, coef normalised as
+ * BASIS::Shell does (hfscf: src/integrals/hfscf_shell.cpp:44-54); cart_offset / pure_offset as
+ * Shell::get_idx / get_ids. atom_of_shell indexes the geometry passed to hfg_set_geometry. */
+int hfg_set_basis(hfg_ctx* ctx, int nshell, const int* L, const int* atom_of_shell, const int* nprim,
+                  const double* alpha, const double* coef, int pure, const int* cart_offset,
+                  const int* pure_offset);
+
+/* Atom positions in bohr (natom x 3) and charges. Recomputes the primitive-pair data, the
+ * Schwarz diagonal (ab|ab) on the GPU, the pair lists and the work decomposition.
+ * Replaces: ShellPair::set_params + Erios::init_screen/compute_screen_matrix
+ * (hfscf_shellpair.cpp:33-60, hfscf_oseri_screen.cpp:17-181). */
+int hfg_set_geometry(hfg_ctx* ctx, int natom, const double* xyz_bohr, const int* Z);
+
+/* J and K (nbf x nbf) for the half density D. Either output may be NULL. flags reserved (0). */
+int hfg_build_jk(hfg_ctx* ctx, const double* D, double* J, double* K, int flags);
+
+/* Fock matrix F = H + 2J - K. When incremental != 0 and a previous build exists, J and K are
+ * built from D - D_prev and added to the previous two-electron matrix (the FP32 error of the
+ * mixed mode then shrinks with the density change). Replaces: hf_solver::update_fock_matrix's
+ * digestion (hfscf_hf_solver.cpp:261-278). */
+int hfg_fock(hfg_ctx* ctx, const double* H, const double* D, int incremental, double* F);
+
+/* Forget the previous density (next hfg_fock is a full build). */
+int hfg_reset_incremental(hfg_ctx* ctx);
+
+/* E_2e = sum_ij D_ij (2J - K)_ij for the half density D. */
+int hfg_energy(hfg_ctx* ctx, const double* D, double* E2);
+
+/* Backward of hfg_energy: two-electron gradient sum_pqrs [2 D_pq D_rs - D_pr D_qs] (pq|rs)'
+ * (natom x 3, overwritten), contracted by the adjoint derivative-ERI kernel without storing
+ * derivative integrals. Replaces: calc_coulomb_exchange_integral_centers + the dGdX loop of
+ * calc_scf_gradient_rhf (hfscf_grad_ints.cpp:110-151, hfscf_grad_rhf.cpp:121-141). */
+int hfg_energy_backward(hfg_ctx* ctx, const double* D, double* grad);
+
+/* Test hook: output-basis block (s1 s2|s3 s4), laid out [a][b][c][d], no screening.
+ * Replaces: Erios::compute_contracted_shell_quartet for one quartet (hfscf_oseri.cpp:22-246). */
+int hfg_eri_shell_quartet(hfg_ctx* ctx, int s1, int s2, int s3, int s4, double* block);
+
+/* Cartesian Schwarz diagonal (pq|pq), ncart x ncart (hfscf's Erios::screen). */
+int hfg_schwarz(hfg_ctx* ctx, double* screen);
+
+/* Statistics of the last hfg_build_jk / hfg_fock: surviving shell quartets, model FLOPs
+ * (the algorithmic FLOP model over surviving quartets), J/K kernel seconds (HIP events). */
+typedef struct hfg_stats {
+    long long quartets;
+    double model_flops;
+    double kernel_seconds;
+    double total_seconds;
+    long long work_items;
+    long long slab_bytes;
+} hfg_stats;
+int hfg_last_stats(const hfg_ctx* ctx, hfg_stats* st);
+
+int hfg_nbf(const hfg_ctx* ctx);
+const char* hfg_last_error(const hfg_ctx* ctx);
+void hfg_destroy(hfg_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HFGPU_H */

@@ -1,0 +1,96 @@
+/*
+ * hfscf host layer — C ABI over the C++ RHF driver (basis, one-electron integrals, DIIS,
+ * SCF loop and gradient assembly) whose two-electron work runs on the GPU through hfgpu.h.
+ *
+ * Mirrors hfscf's run-time seams:
+ *   hfscf_mol_*        MOLEC::Molecule + BASIS::Basisset (hfscf: src/molecule/hfscf_molecule.cpp:21-336,
+ *                      src/basis/hfscf_basis.cpp:19-275)
+ *   hfscf_rhf_run      Mol::hf_solver::init_data + scf_run (hfscf: src/scfsolver/hfscf_hf_solver.cpp:79-476)
+ *   hfscf_rhf_gradient Mol::scf_gradient::calc_scf_gradient_rhf (hfscf: src/gradient/hfscf_grad_rhf.cpp:8-178)
+ * Status 0 = success, negative = error (message from hfscf_last_error()).
+ */
+#ifndef HFSCF_H
+#define HFSCF_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct hfscf_mol hfscf_mol;
+
+/* Two-electron backend. backend = HFSCF_BACKEND_GPU uses libhfgpu; HFSCF_BACKEND_CALLBACK uses
+ * the user functions below (the run-time seam an external Fock builder plugs into). */
+enum { HFSCF_BACKEND_GPU = 0, HFSCF_BACKEND_CALLBACK = 1 };
+
+typedef struct hfscf_callbacks {
+    void* user;
+    /* J and K (nbf x nbf) of half density D */
+    int (*build_jk)(void* user, const double* D, double* J, double* K);
+    /* natom x 3 two-electron gradient, overwritten */
+    int (*grad2e)(void* user, const double* D, double* grad);
+} hfscf_callbacks;
+
+typedef struct hfscf_scf_options {
+    int backend;          /* HFSCF_BACKEND_* */
+    int device;           /* HIP device for the GPU backend */
+    int precision;        /* hfg_precision */
+    int incremental;      /* delta-D Fock builds */
+    int diis_range;       /* hfscf diis_size (vectors kept = range + 1) */
+    int max_iter;
+    int schwarz;          /* Schwarz screening on/off */
+    int verbose;
+    double energy_tol;
+    double rms_tol;
+    double integral_tol;
+    int final_fp64;       /* mixed precision: finish with FP64 builds until converged again */
+    int reserved[7];
+} hfscf_scf_options;
+
+typedef struct hfscf_scf_result {
+    double energy;        /* total RHF energy (Eh) */
+    double energy_elec;
+    double energy_nuc;
+    int iterations;
+    int converged;
+    double fock_seconds;  /* wall time inside Fock builds */
+    double last_fock_seconds;
+    int fock_builds;
+} hfscf_scf_result;
+
+void hfscf_default_scf_options(hfscf_scf_options* o);
+
+/* Molecule from atomic numbers and bohr coordinates, with a Psi4 .gbs basis file.
+ * pure_override: -1 basis-file header, 0 Cartesian, 1 spherical. */
+hfscf_mol* hfscf_mol_create(int natom, const int* Z, const double* xyz_bohr, const char* gbs_path, int pure_override);
+/* Same from an .xyz file in angstrom. */
+hfscf_mol* hfscf_mol_from_xyz(const char* xyz_path, const char* gbs_path, int pure_override);
+void hfscf_mol_destroy(hfscf_mol* m);
+int hfscf_mol_info(const hfscf_mol* m, int* natom, int* nshell, int* nbf, int* ncart, int* nelec, int* pure);
+/* Shell table (arrays sized by nshell, primitive arrays by the total primitive count). */
+int hfscf_mol_nprim_total(const hfscf_mol* m);
+int hfscf_mol_shells(const hfscf_mol* m, int* L, int* atom, int* nprim, int* cart_off, int* pure_off, double* alpha,
+                     double* coef_norm, double* coef_raw);
+int hfscf_mol_geometry(const hfscf_mol* m, int* Z, double* xyz_bohr);
+int hfscf_mol_set_geometry(hfscf_mol* m, const double* xyz_bohr);
+double hfscf_mol_enuc(const hfscf_mol* m);
+
+/* One-electron matrices (nbf x nbf). */
+int hfscf_oneint(const hfscf_mol* m, double* S, double* T, double* V);
+/* 2 D.d(T+V) - 2 Q.dS + nuclear repulsion gradient (natom x 3, overwritten). */
+int hfscf_oneint_gradient(const hfscf_mol* m, const double* D, const double* Q, double* grad);
+
+/* RHF SCF (CORE guess). D and F (nbf x nbf) receive the converged half density and the last
+ * extrapolated Fock matrix; either may be NULL. energies (max_iter) may be NULL. */
+int hfscf_rhf_run(hfscf_mol* m, const hfscf_scf_options* o, const hfscf_callbacks* cb, hfscf_scf_result* res,
+                  double* D, double* F, double* energies);
+
+/* RHF gradient (natom x 3) from converged D and F. */
+int hfscf_rhf_gradient(hfscf_mol* m, const hfscf_scf_options* o, const hfscf_callbacks* cb, const double* D,
+                       const double* F, double* grad);
+
+const char* hfscf_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HFSCF_H */

@@ -1,0 +1,59 @@
+"""The C-ABI libraries load and export every function the headers declare (CPU, no GPU calls
+beyond device discovery)."""
+import os
+import re
+
+import pytest
+
+import hfscf_amd as H
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared(header):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"\b((?:hfg|hfscf)_[a-z0-9_]+)\s*\(", text)
+    return sorted(set(names))
+
+
+@pytest.mark.parametrize("header,loader", [("hfgpu.h", H.hfgpu), ("hfscf.h", H.hfscf)])
+def test_exports(header, loader):
+    lib = loader()
+    names = _declared(header)
+    assert len(names) > 8
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_create_without_gpu_fails_cleanly():
+    import ctypes as C
+    if os.path.exists("/dev/kfd"):
+        pytest.skip("a GPU is present")
+    lib = H.hfgpu()
+    h = C.c_void_p()
+    assert lib.hfg_create(C.byref(h), 0, None) == -4  # HFG_ERR_UNSUPPORTED
+    with pytest.raises(RuntimeError):
+        H.GpuContext(H.Molecule.from_xyz(os.path.join(ROOT, "tests", "golden", "h2o.xyz"), "sto-3g"))
+
+
+def test_shell_table_matches_independent_reader():
+    from oracle import basis_ref
+    for xyz, basis in [("h2o_8.xyz", "cc-pvdz"), ("c12h26.xyz", "6-31g**"), ("h2o.xyz", "6-31g*")]:
+        mol = H.Molecule.from_xyz(os.path.join(ROOT, "tests", "golden", xyz), basis)
+        Z, X = basis_ref.read_xyz(os.path.join(ROOT, "tests", "golden", xyz))
+        ref, pure, nbf = basis_ref.shells_for(Z, basis)
+        sh = mol.shells()
+        assert mol.pure == pure and mol.nbf == nbf
+        for k in ["L", "atom", "nprim", "cart_off", "pure_off"]:
+            assert (sh[k] == ref[k]).all(), k
+        assert abs(sh["coef"] - ref["coef"]).max() < 1e-13
+        assert abs(mol.geometry()[1] - X).max() < 1e-12
+
+
+def test_config_sizes():
+    # basis-function counts of the workloads (cc-pVDZ pure: 192 / 384; 6-31G** Cartesian: 310)
+    sizes = {("h2o_8.xyz", "cc-pvdz"): 192, ("h2o_16.xyz", "cc-pvdz"): 384, ("c12h26.xyz", "6-31g**"): 310,
+             ("h2o.xyz", "sto-3g"): 7}
+    for (xyz, basis), n in sizes.items():
+        assert H.Molecule.from_xyz(os.path.join(ROOT, "tests", "golden", xyz), basis).nbf == n
