@@ -195,6 +195,7 @@ int setup_geometry(hfg_ctx* c) {
         }
     c->pairs.clear();
     c->psmean.clear();
+    c->psq.clear();
     c->bp_off.assign(nbp, 0);
     c->bp_cnt.assign(nbp, 0);
     for (int P = 0; P < nbp; ++P) {
@@ -203,7 +204,18 @@ int setup_geometry(hfg_ctx* c) {
         c->bp_off[P] = (int)c->pairs.size();
         for (auto& e : v) {
             if (e.first * smax < c->tol) continue;
-            c->pairs.push_back(e.second);
+            PairDev pd = e.second;
+            const int ma = (c->L[pd.sa] + 1) * (c->L[pd.sa] + 2) / 2, mb = (c->L[pd.sb] + 1) * (c->L[pd.sb] + 2) / 2;
+            pd.sq_off = (int)c->psq.size();
+            pd.sq_min = 1e300;
+            for (int x = 0; x < ma; ++x)
+                for (int y = 0; y < mb; ++y) {
+                    const double q = std::sqrt(
+                        std::fabs(c->screen[(size_t)(c->cart_off[pd.sa] + x) * c->ncart + c->cart_off[pd.sb] + y]));
+                    c->psq.push_back(q);
+                    pd.sq_min = std::min(pd.sq_min, q);
+                }
+            c->pairs.push_back(pd);
             c->psmean.push_back(e.first);
         }
         c->bp_cnt[P] = (int)c->pairs.size() - c->bp_off[P];
@@ -368,6 +380,7 @@ int setup_geometry(hfg_ctx* c) {
     std::vector<int> shell_atom(c->atom);
     HFG_CHECK(c, c->d_pairs.upload(c->pairs, c->stream));
     HFG_CHECK(c, c->d_psmean.upload(c->psmean, c->stream));
+    HFG_CHECK(c, c->d_psq.upload(c->psq, c->stream));
     HFG_CHECK(c, c->d_bp_off.upload(c->bp_off, c->stream));
     HFG_CHECK(c, c->d_bp_cnt.upload(c->bp_cnt, c->stream));
     HFG_CHECK(c, c->d_blk_nf.upload(c->blk_nf, c->stream));
@@ -408,6 +421,8 @@ int run_jk(hfg_ctx* c) {
     a.D = c->d_D.p;
     a.n = c->nbf;
     a.tol = c->tol;
+    a.psq = c->d_psq.p;
+    a.elem_tol = c->tol;
     a.prim_cut = c->opt.integral_tol / 1e4;
     a.slab = c->d_slab.p;
     HFG_CHECK(c, hipEventRecord(c->ev0, c->stream));
@@ -658,6 +673,7 @@ void hfg_destroy(hfg_ctx* c) {
     c->d_G.release();
     c->d_slab.release();
     c->d_psmean.release();
+    c->d_psq.release();
     c->d_grad_slab.release();
     c->d_grad.release();
     c->d_pp0.release();

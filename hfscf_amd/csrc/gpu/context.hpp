@@ -78,6 +78,7 @@ struct hfg_ctx {
     // pairs
     std::vector<PairDev> pairs;
     std::vector<double> psmean;
+    std::vector<double> psq;  // sqrt|(pq|pq)| per pair component
     std::vector<int> bp_off, bp_cnt;
     std::vector<double4> pp0, pp1, pp2;
     std::vector<double> screen;
@@ -90,7 +91,7 @@ struct hfg_ctx {
     long long slab_doubles = 0, total_quartets = 0, total_items = 0;
     double total_flops = 0, total_grad_flops = 0;
     // device
-    DevBuf<double> d_boys, d_D, d_J, d_K, d_F, d_H, d_G, d_slab, d_psmean, d_grad_slab, d_grad;
+    DevBuf<double> d_psq, d_boys, d_D, d_J, d_K, d_F, d_H, d_G, d_slab, d_psmean, d_grad_slab, d_grad;
     DevBuf<double4> d_pp0, d_pp1, d_pp2;
     DevBuf<PairDev> d_pairs;
     DevBuf<int> d_bp_off, d_bp_cnt, d_blk_nf, d_blk_func_off, d_blk_func_ao, d_gJ_off, d_gK_off, d_shell_atom;

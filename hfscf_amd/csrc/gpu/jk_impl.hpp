@@ -71,7 +71,7 @@ __global__ __launch_bounds__(256) void jk_kernel(JkArgs a) {
         if (kp.sa == kp.sb) s *= 0.5;
         if (same && i == j) s *= 0.5;
         R I[NA * NB * NC * ND];
-        eri_block<LA, LB, LC, LD, PURE, R>(bp, kp, a.pp0, a.pp1, a.pp2, a.boys, a.prim_cut, I);
+        eri_block<LA, LB, LC, LD, PURE, R>(bp, kp, a.pp0, a.pp1, a.pp2, a.boys, a.prim_cut, a.psq, a.elem_tol, I);
         digest<NA, NB, NC, ND, R, ACC>(I, s, a.D, a.n, bp, kp, tJab, tJcd, tKac, tKad, tKbc, tKbd, fB, fC, fD);
     }
     __syncthreads();

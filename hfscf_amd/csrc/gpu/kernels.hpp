@@ -19,6 +19,8 @@ struct JkArgs {
     const double* boys;
     const double* D;
     int n;
+    const double* psq;  // sqrt|(pq|pq)| per pair component (element screen)
+    double elem_tol;    // element screen threshold (0 = off)
     double tol;        // shell-quartet Schwarz threshold on mean values
     double prim_cut;   // primitive-quartet prefactor cutoff
     double* slab;      // FP64 tiles per work item

@@ -22,6 +22,8 @@ struct PairDev {
     int off, np;       // primitive-pair range
     int slot_a, slot_b;  // positions of sa / sb inside their blocks
     int ao_a, ao_b;    // first output-basis function of sa / sb
+    int sq_off;        // offset of sqrt|(pq|pq)| per Cartesian component pair (p in sa, q in sb)
+    double sq_min;     // smallest of those values
     double AB[3];      // A - B
 };
 

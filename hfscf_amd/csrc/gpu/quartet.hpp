@@ -60,18 +60,44 @@ HFG_INLINE void eri_cart(const PairDev& bp, const PairDev& kp, const double4* __
     H::run(acc, AB, CD, out);
 }
 
+// hfscf's element screen: a Cartesian component (pq|rs) whose Schwarz bound
+// sqrt|(pq|pq)| sqrt|(rs|rs)| is below the threshold is dropped before the spherical transform
+// (hfscf: src/integrals/hfscf_oseri.cpp:152-158). sq holds sqrt|(pq|pq)| per pair; the mask is
+// only evaluated when the smallest bound of the quartet is below the threshold.
+template <int LA, int LB, int LC, int LD, typename R>
+HFG_INLINE void element_screen(const PairDev& bp, const PairDev& kp, const double* __restrict__ sq, double tol,
+                               R* cart) {
+    if (tol <= 0.0 || bp.sq_min * kp.sq_min >= tol) return;
+    constexpr int na = ncart(LA), nb = ncart(LB), nc = ncart(LC), nd = ncart(LD);
+    const double* sb = sq + bp.sq_off;
+    const double* sk = sq + kp.sq_off;
+    // Written as a multiplication by a 0/1 factor: the equivalent conditional store was
+    // mis-generated for the (dd|dp) class on gfx950 (its spill-heavy kernel produced wrong
+    // integrals), while this form gives the same results as a CPU build of the same code.
+    sfor<na * nb>([&](auto ab) {
+        const double x = sb[ab];
+        sfor<nc * nd>([&](auto cd) {
+            const R keep = (x * sk[cd] >= tol) ? (R)1 : (R)0;
+            cart[ab * (nc * nd) + cd] *= keep;
+        });
+    });
+}
+
 // Output-basis block of one quartet (spherical when PURE), out[a][b][c][d].
 template <int LA, int LB, int LC, int LD, bool PURE, typename R>
 HFG_INLINE void eri_block(const PairDev& bp, const PairDev& kp, const double4* __restrict__ pp0,
                           const double4* __restrict__ pp1, const double4* __restrict__ pp2,
-                          const double* __restrict__ boys_tab, double prim_cut, R* out) {
+                          const double* __restrict__ boys_tab, double prim_cut, const double* __restrict__ sq,
+                          double tol, R* out) {
     using S = Sph<LA, LB, LC, LD, PURE, R>;
     if constexpr (S::any) {
         R cart[S::c0 * S::c1 * S::c2 * S::c3];
         eri_cart<LA, LB, LC, LD, R>(bp, kp, pp0, pp1, pp2, boys_tab, prim_cut, cart);
+        element_screen<LA, LB, LC, LD, R>(bp, kp, sq, tol, cart);
         S::run(cart, out);
     } else {
         eri_cart<LA, LB, LC, LD, R>(bp, kp, pp0, pp1, pp2, boys_tab, prim_cut, out);
+        element_screen<LA, LB, LC, LD, R>(bp, kp, sq, tol, out);
     }
 }
 
