@@ -5,11 +5,14 @@ digested and summed on the GPU) for the converged density of the workload. Defau
 (H2O)8 / cc-pVDZ (pure d, 192 basis functions), Schwarz threshold 1e-12, FP64.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload h2o8|c12h26|h2o16]
-                    [--no-cpu] [--no-extra]
+                    [--no-cpu] [--no-extra] [--no-mixed]
 
 Prints one JSON line (value = FP64 Fock builds/s on the GPU) with a roofline object for the
 J/K kernels (model FLOPs of the surviving quartets / HIP-event kernel time, against the FP64
 vector peak) and the CPU baseline (the oracle, timed on a bounded sample on this host).
+"extra" holds the other figures of the metric: two-electron gradients/s (adjoint kernel),
+the mixed-precision Fock build and gradient rates at the same density with their errors
+against FP64, and whole-SCF timings of the FP64 and mixed (incremental) solvers with |dE|.
 """
 import argparse
 import json
@@ -26,11 +29,17 @@ import hfscf_amd as H  # noqa: E402
 
 METRIC = "Fock builds/s + gradients/s, mixed vs FP64 at |dE|<=1e-6 Eh; % of VALU peak"
 PEAK_FP64 = 78.6  # TFLOP/s, MI355X FP64 vector
+PEAK_FP32 = 157.3  # TFLOP/s, MI355X FP32 vector (packed; 78.6 without packed FP32)
 WORKLOADS = {
     "h2o8": ("h2o_8.xyz", "cc-pvdz", "(H2O)8 / cc-pVDZ (pure d), 192 bf, direct RHF Fock build, Schwarz 1e-12"),
     "c12h26": ("c12h26.xyz", "6-31g**", "C12H26 / 6-31G** (Cartesian d), 310 bf, direct RHF Fock build"),
     "h2o16": ("h2o_16.xyz", "cc-pvdz", "(H2O)16 / cc-pVDZ (pure d), 384 bf, direct RHF Fock build"),
 }
+SCF_OPTS = dict(diis_range=8, energy_tol=1e-10, rms_tol=1e-10)
+
+
+def log(msg):
+    print("# " + msg, file=sys.stderr, flush=True)
 
 
 def time_builds(ctx, Hc, D, steps, warmup):
@@ -43,6 +52,32 @@ def time_builds(ctx, Hc, D, steps, warmup):
         kern += ctx.stats()["kernel_seconds"]
     el = time.perf_counter() - t0
     return el, kern
+
+
+def time_grads(ctx, D, steps, warmup):
+    g = None
+    for _ in range(warmup):
+        g = ctx.energy_backward(D)
+    kern = 0.0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        g = ctx.energy_backward(D)
+        kern += ctx.stats()["kernel_seconds"]
+    return time.perf_counter() - t0, kern, g
+
+
+def copy_calibration(mib, reps=3):
+    """A plain device-to-device copy of a known size, so that a profiler's HBM byte counters can
+    be scaled against it."""
+    import torch
+
+    n = int(mib * 2 ** 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    log(f"copy calibration: {reps} copies of {n * 4} bytes (reads {n * 4} B, writes {n * 4} B each)")
 
 
 def cpu_baseline(mol, budget_rows=32):
@@ -80,6 +115,46 @@ def cpu_baseline(mol, budget_rows=32):
     }
 
 
+def extra_figures(mol, Hc, D, ref_F, ref_g, scf64, steps, warmup, mixed):
+    out = {}
+    ctx = H.GpuContext(mol)
+    el, kern, g64 = time_grads(ctx, D, steps, warmup)
+    st = ctx.stats()
+    out["fp64_gradients_per_s"] = steps / el
+    out["fp64_gradient_kernel_ms"] = 1e3 * kern / steps
+    out["fp64_gradient_model_tflops"] = st["model_flops"] / (kern / steps) / 1e12
+    log(f"FP64 gradient: {steps / el:.2f}/s, kernel {1e3 * kern / steps:.2f} ms")
+    ctx.close()
+    if not mixed:
+        return out
+    ctxm = H.GpuContext(mol, precision=H.MIXED)
+    el, kern = time_builds(ctxm, Hc, D, steps, warmup)
+    Fm = ctxm.fock(Hc, D)
+    out["mixed_fock_builds_per_s"] = steps / el
+    out["mixed_fock_kernel_ms"] = 1e3 * kern / steps
+    out["mixed_fock_max_abs_err_vs_fp64"] = float(np.abs(Fm - ref_F).max())
+    out["mixed_energy_err_at_fixed_D"] = float(np.sum(D * (Fm - ref_F)))
+    el, kern, gm = time_grads(ctxm, D, steps, warmup)
+    out["mixed_gradients_per_s"] = steps / el
+    out["mixed_gradient_kernel_ms"] = 1e3 * kern / steps
+    out["mixed_gradient_max_abs_err_vs_fp64"] = float(np.abs(gm - g64).max())
+    log(f"mixed build {out['mixed_fock_builds_per_s']:.2f}/s, gradient {out['mixed_gradients_per_s']:.2f}/s")
+    ctxm.close()
+    # whole SCF: FP64 (full builds, as hfscf) vs mixed (FP32 incremental builds + final FP64)
+    t0 = time.perf_counter()
+    scfm = H.rhf(mol, precision=H.MIXED, incremental=1, **SCF_OPTS)
+    tm = time.perf_counter() - t0
+    out["scf_fp64_seconds"] = scf64["seconds"]
+    out["scf_fp64_fock_seconds"] = scf64["fock_seconds"]
+    out["scf_fp64_iterations"] = scf64["iterations"]
+    out["scf_mixed_seconds"] = tm
+    out["scf_mixed_fock_seconds"] = scfm["fock_seconds"]
+    out["scf_mixed_iterations"] = scfm["iterations"]
+    out["scf_mixed_dE_vs_fp64"] = scfm["energy_final"] - scf64["energy_final"]
+    log(f"SCF mixed: {tm:.2f} s (Fock {scfm['fock_seconds']:.2f} s), dE {out['scf_mixed_dE_vs_fp64']:.2e}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -88,6 +163,12 @@ def main():
     ap.add_argument("--workload", default="h2o8", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-scf", action="store_true",
+                    help="time the builds on a seeded random density instead of the converged one "
+                         "(FP64 build cost does not depend on the density values; for profiler runs)")
+    ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="MIB",
+                    help="also run a plain device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (see DESIGN.md, sharding)",
@@ -97,19 +178,30 @@ def main():
     mol = H.Molecule.from_xyz(os.path.join(ROOT, "tests", "golden", xyz), basis)
     S, T, V = mol.oneint()
     Hc = T + V
-    # converged density from a GPU SCF (untimed)
-    t0 = time.perf_counter()
-    scf = H.rhf(mol, diis_range=8)
-    t_scf = time.perf_counter() - t0
-    D = scf["D"]
-    print(f"# SCF {args.workload}: E = {scf['energy']:.10f} Eh, {scf['iterations']} iterations, "
-          f"{t_scf:.2f} s", file=sys.stderr, flush=True)
+    if args.copy_calibration > 0:
+        copy_calibration(args.copy_calibration)
+    if args.no_scf:
+        rng = np.random.default_rng(7)
+        A = rng.standard_normal((mol.nbf, mol.nbf))
+        D = 0.05 * (A + A.T)
+        scf = dict(energy=float("nan"), energy_final=float("nan"), iterations=0, seconds=0.0, fock_seconds=0.0)
+    else:
+        # converged density from a GPU SCF (untimed)
+        t0 = time.perf_counter()
+        scf = H.rhf(mol, **SCF_OPTS)
+        scf["seconds"] = time.perf_counter() - t0
+        D = scf["D"]
+        log(f"SCF {args.workload}: E = {scf['energy']:.10f} Eh (final build {scf['energy_final']:.10f}), "
+            f"{scf['iterations']} iterations, {scf['seconds']:.2f} s")
     ctx = H.GpuContext(mol)
     el, kern = time_builds(ctx, Hc, D, args.steps, args.warmup)
     st = ctx.stats()
+    F64 = ctx.fock(Hc, D)
+    ctx.close()
     builds_s = args.steps / el
     kern_per = kern / args.steps
     achieved = st["model_flops"] / kern_per / 1e12
+    log(f"FP64 Fock build: {builds_s:.2f}/s, kernel {1e3 * kern_per:.2f} ms, {achieved:.2f} TFLOP/s model")
     out = {
         "metric": METRIC,
         "value": builds_s,
@@ -124,14 +216,21 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": desc, "nbf": mol.nbf, "shell_quartets": st["quartets"], "precision": "fp64",
-                   "scf_energy": scf["energy"], "scf_iterations": scf["iterations"], "scf_seconds": t_scf},
+                   "scf_energy": scf["energy"], "scf_energy_final": scf["energy_final"],
+                   "scf_iterations": scf["iterations"], "scf_seconds": scf["seconds"],
+                   "quartets_per_s": st["quartets"] * builds_s},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP64, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64, "traffic": None,
                      "model_gflop_per_build": st["model_flops"] / 1e9, "kernel_ms_per_build": 1e3 * kern_per},
     }
     ref_path = os.path.join(ROOT, "tests", "golden", "cfg2_h2o8_ccpvdz.npz")
     if args.workload == "h2o8" and os.path.exists(ref_path):
-        out["config"]["dE_vs_oracle"] = scf["energy"] - float(np.load(ref_path)["energy"])
+        f = np.load(ref_path)
+        out["config"]["dE_vs_oracle"] = scf["energy"] - float(f["energy"])
+        if "energy_final" in f:
+            out["config"]["dE_final_vs_oracle"] = scf["energy_final"] - float(f["energy_final"])
+    if not args.no_extra:
+        out["extra"] = extra_figures(mol, Hc, D, F64, None, scf, max(2, args.steps // 2), 1, not args.no_mixed)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(mol)
     print(json.dumps(out), flush=True)

@@ -28,7 +28,7 @@ FP64, MIXED = 0, 1
 
 class HfgOptions(C.Structure):
     _fields_ = [("precision", C.c_int), ("integral_tol", C.c_double), ("schwarz", C.c_int),
-                ("reserved", C.c_int * 8)]
+                ("density_tol", C.c_double), ("reserved", C.c_int * 6)]
 
 
 class HfgStats(C.Structure):
@@ -40,13 +40,13 @@ class ScfOptions(C.Structure):
     _fields_ = [("backend", C.c_int), ("device", C.c_int), ("precision", C.c_int), ("incremental", C.c_int),
                 ("diis_range", C.c_int), ("max_iter", C.c_int), ("schwarz", C.c_int), ("verbose", C.c_int),
                 ("energy_tol", C.c_double), ("rms_tol", C.c_double), ("integral_tol", C.c_double),
-                ("final_fp64", C.c_int), ("reserved", C.c_int * 7)]
+                ("final_fp64", C.c_int), ("density_tol", C.c_double), ("reserved", C.c_int * 5)]
 
 
 class ScfResult(C.Structure):
     _fields_ = [("energy", C.c_double), ("energy_elec", C.c_double), ("energy_nuc", C.c_double),
                 ("iterations", C.c_int), ("converged", C.c_int), ("fock_seconds", C.c_double),
-                ("last_fock_seconds", C.c_double), ("fock_builds", C.c_int)]
+                ("last_fock_seconds", C.c_double), ("fock_builds", C.c_int), ("energy_final", C.c_double)]
 
 
 JKFUNC = C.CFUNCTYPE(C.c_int, C.c_void_p, _dp, _dp, _dp)
@@ -80,6 +80,7 @@ def hfgpu():
         lib.hfg_build_jk.argtypes = [C.c_void_p, _dp, _dp, _dp, C.c_int]
         lib.hfg_fock.argtypes = [C.c_void_p, _dp, _dp, C.c_int, _dp]
         lib.hfg_reset_incremental.argtypes = [C.c_void_p]
+        lib.hfg_set_precision.argtypes = [C.c_void_p, C.c_int]
         lib.hfg_energy.argtypes = [C.c_void_p, _dp, _dp]
         lib.hfg_energy_backward.argtypes = [C.c_void_p, _dp, _dp]
         lib.hfg_eri_shell_quartet.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, _dp]
@@ -246,7 +247,7 @@ def rhf(mol, jk=None, grad2e=None, **opts):
     rc = lib.hfscf_rhf_run(mol._h, C.byref(o), C.byref(cb) if cb else None, C.byref(res), _ptr(D), _ptr(F), _ptr(E))
     if rc not in (0, -2):
         raise RuntimeError(lib.hfscf_last_error().decode())
-    return dict(energy=res.energy, energy_elec=res.energy_elec, energy_nuc=res.energy_nuc, iterations=res.iterations,
+    return dict(energy=res.energy, energy_final=res.energy_final, energy_elec=res.energy_elec, energy_nuc=res.energy_nuc, iterations=res.iterations,
                 converged=bool(res.converged), fock_seconds=res.fock_seconds, D=D, F=F,
                 energies=E[:res.iterations + 1].copy())
 
@@ -270,11 +271,12 @@ def rhf_gradient(mol, D, F, jk=None, grad2e=None, **opts):
 class GpuContext:
     """The GPU two-electron engine for one molecule (hfgpu C ABI)."""
 
-    def __init__(self, mol, precision=FP64, integral_tol=1e-14, schwarz=True, device=0):
+    def __init__(self, mol, precision=FP64, integral_tol=1e-14, schwarz=True, device=0, density_tol=0.0):
         lib = hfgpu()
         opt = HfgOptions()
         lib.hfg_default_options(C.byref(opt))
         opt.precision, opt.integral_tol, opt.schwarz = precision, integral_tol, int(schwarz)
+        opt.density_tol = density_tol
         h = C.c_void_p()
         rc = lib.hfg_create(C.byref(h), device, C.byref(opt))
         if rc:
@@ -312,6 +314,9 @@ class GpuContext:
         D = np.ascontiguousarray(D, dtype=np.float64)
         self._check(hfgpu().hfg_fock(self._h, _ptr(H), _ptr(D), int(incremental), _ptr(F)))
         return F
+
+    def set_precision(self, precision):
+        self._check(hfgpu().hfg_set_precision(self._h, int(precision)))
 
     def energy(self, D):
         e = C.c_double()

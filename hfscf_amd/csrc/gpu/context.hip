@@ -48,11 +48,15 @@ void build_blocks(hfg_ctx* c) {
     c->shell_slot.assign(c->nshell, -1);
     size_t k = 0;
     while (k < order.size()) {
-        const int Lb = c->L[order[k]];
+        // one angular momentum and one primitive count per block: every shell pair of a block
+        // pair then has the same number of primitive pairs, so the lanes of a wavefront run
+        // primitive loops of equal length
+        const int Lb = c->L[order[k]], np = c->nprim[order[k]];
         const int nf = nfun(Lb, c->pure);
         const int per = std::max(1, kBlockFuncs / nf);
         std::vector<int> sh;
-        while (k < order.size() && c->L[order[k]] == Lb && (int)sh.size() < per) sh.push_back(order[k++]);
+        while (k < order.size() && c->L[order[k]] == Lb && c->nprim[order[k]] == np && (int)sh.size() < per)
+            sh.push_back(order[k++]);
         const int b = (int)c->blk_L.size();
         for (size_t s = 0; s < sh.size(); ++s) {
             c->shell_blk[sh[s]] = b;
@@ -208,12 +212,14 @@ int setup_geometry(hfg_ctx* c) {
             const int ma = (c->L[pd.sa] + 1) * (c->L[pd.sa] + 2) / 2, mb = (c->L[pd.sb] + 1) * (c->L[pd.sb] + 2) / 2;
             pd.sq_off = (int)c->psq.size();
             pd.sq_min = 1e300;
+            pd.sq_max = 0.0;
             for (int x = 0; x < ma; ++x)
                 for (int y = 0; y < mb; ++y) {
                     const double q = std::sqrt(
                         std::fabs(c->screen[(size_t)(c->cart_off[pd.sa] + x) * c->ncart + c->cart_off[pd.sb] + y]));
                     c->psq.push_back(q);
                     pd.sq_min = std::min(pd.sq_min, q);
+                    pd.sq_max = std::max(pd.sq_max, q);
                 }
             c->pairs.push_back(pd);
             c->psmean.push_back(e.first);
@@ -249,8 +255,9 @@ int setup_geometry(hfg_ctx* c) {
         c->classes.push_back(w);
         return (int)c->classes.size() - 1;
     };
-    // pass 1: total cost
+    // pass 1: total cost and cost per class
     double total_cost = 0;
+    std::vector<double> class_cost;
     auto bpX = [&](int P) {
         int X = 0;
         while ((X + 1) * (X + 2) / 2 <= P) ++X;
@@ -261,11 +268,16 @@ int setup_geometry(hfg_ctx* c) {
             if (!c->bp_cnt[P] || !c->bp_cnt[Q]) continue;
             const int A = bpX(P), B = P - A * (A + 1) / 2, C = bpX(Q), D = Q - C * (C + 1) / 2;
             const FlopParts fp = model_flops(c->blk_L[A], c->blk_L[B], c->blk_L[C], c->blk_L[D], c->pure, false);
+            const int ci = class_index(c->blk_L[A], c->blk_L[B], c->blk_L[C], c->blk_L[D]);
+            if ((int)class_cost.size() <= ci) class_cost.resize(ci + 1, 0.0);
             for (int i = 0; i < c->bp_cnt[P]; ++i) {
                 int cnt = ket_count_host(&c->psmean[c->bp_off[Q]], c->bp_cnt[Q], c->psmean[c->bp_off[P] + i], c->tol);
                 if (P == Q) cnt = std::min(cnt, i + 1);
                 if (!cnt) continue;
-                total_cost += c->pairs[c->bp_off[P] + i].np * cum_np[cum_off[Q] + cnt] * fp.per_prim + cnt * fp.per_quartet;
+                const double cost =
+                    c->pairs[c->bp_off[P] + i].np * cum_np[cum_off[Q] + cnt] * fp.per_prim + cnt * fp.per_quartet;
+                total_cost += cost;
+                class_cost[ci] += cost;
             }
         }
     const double target = std::max(total_cost / (256.0 * 6.0), 1.0);
@@ -283,9 +295,13 @@ int setup_geometry(hfg_ctx* c) {
             const FlopParts gp = model_flops(cw.la, cw.lb, cw.lc, cw.ld, c->pure, true);
             const int tile = c->blk_nf[A] * c->blk_nf[B] + c->blk_nf[C] * c->blk_nf[D] + c->blk_nf[A] * c->blk_nf[C] +
                              c->blk_nf[A] * c->blk_nf[D] + c->blk_nf[B] * c->blk_nf[C] + c->blk_nf[B] * c->blk_nf[D];
+            // per-class item target: a class launch should have >= 256 items when it has the work
+            const double tgt = std::max(std::min(target, class_cost[ci] / 256.0), 1.0);
             WorkItem w{};
             w.P = P;
             w.Q = Q;
+            w.c0 = 0;
+            w.c1 = 0x7fffffff;
             w.A = A;
             w.B = B;
             w.C = C;
@@ -302,6 +318,8 @@ int setup_geometry(hfg_ctx* c) {
                 }
                 w.r0 = r0;
                 w.r1 = r1;
+                w.c0 = 0;
+                w.c1 = 0x7fffffff;
                 w.slab = c->slab_doubles;
                 c->slab_doubles += tile;
                 cw.items.push_back(w);
@@ -317,10 +335,28 @@ int setup_geometry(hfg_ctx* c) {
                 cw.quartets += cnt;
                 cw.flops += kq * fp.per_prim + cnt * fp.per_quartet;
                 cw.grad_flops += kq * gp.per_prim + cnt * gp.per_quartet;
+                const double row_cost = kq * fp.per_prim + cnt * fp.per_quartet;
+                if (row_cost > tgt && cnt > 64) {
+                    // an expensive row becomes several items over ket column chunks
+                    flush(i);
+                    const int chunks = std::min((int)std::ceil(row_cost / tgt), (cnt + 31) / 32);
+                    const int width = (cnt + chunks - 1) / chunks;
+                    for (int j0 = 0; j0 < cnt; j0 += width) {
+                        WorkItem v = w;
+                        v.r0 = i;
+                        v.r1 = i + 1;
+                        v.c0 = j0;
+                        v.c1 = std::min(cnt, j0 + width);
+                        v.slab = c->slab_doubles;
+                        c->slab_doubles += tile;
+                        cw.items.push_back(v);
+                    }
+                    continue;
+                }
                 if (r0 < 0) r0 = i;
-                acc += kq * fp.per_prim + cnt * fp.per_quartet;
+                acc += row_cost;
                 q_acc += cnt;
-                if (acc >= target || i + 1 - r0 >= kMaxRows) flush(i + 1);
+                if (acc >= tgt || i + 1 - r0 >= kMaxRows) flush(i + 1);
             }
             flush(c->bp_cnt[P]);
         }
@@ -332,6 +368,24 @@ int setup_geometry(hfg_ctx* c) {
     }
     c->classes.erase(std::remove_if(c->classes.begin(), c->classes.end(), [](const ClassWork& w) { return w.items.empty(); }),
                      c->classes.end());
+    // classes to side streams, heaviest first onto the least loaded stream
+    {
+        std::vector<size_t> ord(c->classes.size());
+        std::iota(ord.begin(), ord.end(), 0);
+        std::sort(ord.begin(), ord.end(), [&](size_t x, size_t y) { return c->classes[x].flops > c->classes[y].flops; });
+        std::vector<ClassWork> sorted;
+        for (size_t k : ord) sorted.push_back(c->classes[k]);
+        c->classes.swap(sorted);
+        double load[kSideStreams] = {0};
+        c->class_stream.assign(c->classes.size(), 0);
+        for (size_t k = 0; k < c->classes.size(); ++k) {
+            int best = 0;
+            for (int s = 1; s < kSideStreams; ++s)
+                if (load[s] < load[best]) best = s;
+            c->class_stream[k] = best;
+            load[best] += c->classes[k].flops;
+        }
+    }
     // gather lists
     const int ntile = nbp;
     std::vector<std::vector<GatherEntry>> lj(ntile), lk(ntile);
@@ -406,8 +460,28 @@ int setup_geometry(hfg_ctx* c) {
     return HFG_OK;
 }
 
-// J and K of the density already in d_D, into d_J / d_K.
-int run_jk(hfg_ctx* c) {
+// Largest |D| of every shell block (nshell x nshell), uploaded for the density screen.
+int upload_dmax(hfg_ctx* c, const double* D) {
+    const int ns = c->nshell, n = c->nbf;
+    c->dmax.assign((size_t)ns * ns, 0.0f);
+    for (int a = 0; a < ns; ++a) {
+        const int oa = c->pure ? c->pure_off[a] : c->cart_off[a], na = nfun(c->L[a], c->pure);
+        for (int b = 0; b < ns; ++b) {
+            const int ob = c->pure ? c->pure_off[b] : c->cart_off[b], nb = nfun(c->L[b], c->pure);
+            double m = 0.0;
+            for (int p = 0; p < na; ++p)
+                for (int q = 0; q < nb; ++q) m = std::max(m, std::fabs(D[(size_t)(oa + p) * n + ob + q]));
+            // rounded up so the single-precision table never understates the bound
+            c->dmax[(size_t)a * ns + b] = std::nextafter((float)m, INFINITY);
+        }
+    }
+    HFG_CHECK(c, c->d_dmax.upload(c->dmax, c->stream));
+    return HFG_OK;
+}
+
+// J and K of the density already in d_D, into d_J / d_K. dscreen: apply the density-weighted
+// screen with the table from upload_dmax.
+int run_jk(hfg_ctx* c, bool dscreen = false) {
     JkArgs a{};
     a.pairs = c->d_pairs.p;
     a.psmean = c->d_psmean.p;
@@ -425,11 +499,16 @@ int run_jk(hfg_ctx* c) {
     a.elem_tol = c->tol;
     a.prim_cut = c->opt.integral_tol / 1e4;
     a.slab = c->d_slab.p;
+    a.dmax = c->d_dmax.p;
+    a.nshell = c->nshell;
+    a.dtol = (dscreen && c->d_dmax.p) ? c->opt.density_tol : 0.0;
+    const int mixed = c->opt.precision == HFG_MIXED ? 1 : 0;
     HFG_CHECK(c, hipEventRecord(c->ev0, c->stream));
-    for (auto& cw : c->classes) {
-        a.items = c->d_items.p + cw.dev_off;
-        HFG_CHECK(c, launch_jk(cw.la, cw.lb, cw.lc, cw.ld, c->pure, c->opt.precision, a, (int)cw.items.size(), c->stream));
-    }
+    HFG_CHECK(c, for_each_class_concurrent(c, [&](const ClassWork& cw, hipStream_t st) {
+        JkArgs b = a;
+        b.items = c->d_items.p + cw.dev_off;
+        return launch_jk(cw.la, cw.lb, cw.lc, cw.ld, c->pure, mixed, b, (int)cw.items.size(), st);
+    }));
     HFG_CHECK(c, hipEventRecord(c->ev1, c->stream));
     GatherArgs g{};
     g.tiles = c->d_tiles.p;
@@ -491,10 +570,17 @@ int hfg_create(hfg_ctx** out, int device, const hfg_options* opt) {
     else
         hfg_default_options(&c->opt);
     if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess ||
-        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess) {
         delete c;
         return HFG_ERR_HIP;
     }
+    for (int k = 0; k < kSideStreams; ++k)
+        if (hipStreamCreate(&c->side[k]) != hipSuccess ||
+            hipEventCreateWithFlags(&c->ev_join[k], hipEventDisableTiming) != hipSuccess) {
+            hfg_destroy(c);
+            return HFG_ERR_HIP;
+        }
     std::vector<double> tab = boys_table();
     if (c->d_boys.upload(tab, c->stream) != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess) {
         delete c;
@@ -576,9 +662,11 @@ int hfg_fock(hfg_ctx* c, const double* H, const double* D, int incremental, doub
         for (size_t k = 0; k < nn; ++k) Dd[k] = D[k] - c->Dprev[k];
         Duse = Dd.data();
     }
+    const bool dscreen = inc && c->opt.density_tol > 0.0;
+    if (dscreen && (r = upload_dmax(c, Duse))) return r;
     HFG_CHECK(c, hipMemcpyAsync(c->d_D.p, Duse, nn * sizeof(double), hipMemcpyHostToDevice, c->stream));
     HFG_CHECK(c, hipMemcpyAsync(c->d_H.p, H, nn * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    if ((r = run_jk(c))) return r;
+    if ((r = run_jk(c, dscreen))) return r;
     HFG_CHECK(c, launch_fock(c->d_H.p, c->d_J.p, c->d_K.p, c->d_F.p, c->d_G.p, (int)nn, inc ? 1 : 0, c->stream));
     HFG_CHECK(c, hipMemcpyAsync(F, c->d_F.p, nn * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     c->Dprev.assign(D, D + nn);
@@ -589,6 +677,13 @@ int hfg_fock(hfg_ctx* c, const double* H, const double* D, int incremental, doub
 int hfg_reset_incremental(hfg_ctx* c) {
     if (!c) return HFG_ERR_ARG;
     c->have_prev = false;
+    return HFG_OK;
+}
+
+int hfg_set_precision(hfg_ctx* c, int precision) {
+    if (!c) return HFG_ERR_ARG;
+    if (precision != HFG_FP64 && precision != HFG_MIXED) return fail(c, HFG_ERR_ARG, "hfg_set_precision: unknown mode");
+    c->opt.precision = precision;
     return HFG_OK;
 }
 
@@ -674,6 +769,7 @@ void hfg_destroy(hfg_ctx* c) {
     c->d_slab.release();
     c->d_psmean.release();
     c->d_psq.release();
+    c->d_dmax.release();
     c->d_grad_slab.release();
     c->d_grad.release();
     c->d_pp0.release();
@@ -692,6 +788,12 @@ void hfg_destroy(hfg_ctx* c) {
     c->d_tiles.release();
     c->d_gJ.release();
     c->d_gK.release();
+    for (int k = 0; k < kSideStreams; ++k) {
+        if (c->side[k]) (void)hipStreamSynchronize(c->side[k]);
+        if (c->ev_join[k]) (void)hipEventDestroy(c->ev_join[k]);
+        if (c->side[k]) (void)hipStreamDestroy(c->side[k]);
+    }
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);

@@ -36,6 +36,8 @@ struct DevBuf {
     }
 };
 
+constexpr int kSideStreams = 4;  // class launches run concurrently on these
+
 inline int nfun(int L, bool pure) { return pure ? 2 * L + 1 : (L + 1) * (L + 2) / 2; }
 
 
@@ -60,6 +62,9 @@ struct hfg_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipStream_t side[hfg::kSideStreams] = {};
+    hipEvent_t ev_fork = nullptr, ev_join[hfg::kSideStreams] = {};
+    std::vector<int> class_stream;  // side stream of each class launch
     hfg_options opt{};
     std::string err;
     bool have_basis = false, have_geom = false;
@@ -92,6 +97,8 @@ struct hfg_ctx {
     double total_flops = 0, total_grad_flops = 0;
     // device
     DevBuf<double> d_psq, d_boys, d_D, d_J, d_K, d_F, d_H, d_G, d_slab, d_psmean, d_grad_slab, d_grad;
+    DevBuf<float> d_dmax;  // largest |density| per shell block (density-weighted screening)
+    std::vector<float> dmax;
     DevBuf<double4> d_pp0, d_pp1, d_pp2;
     DevBuf<PairDev> d_pairs;
     DevBuf<int> d_bp_off, d_bp_cnt, d_blk_nf, d_blk_func_off, d_blk_func_ao, d_gJ_off, d_gK_off, d_shell_atom;
@@ -117,4 +124,23 @@ struct hfg_ctx {
 namespace hfg {
 // Two-electron gradient (natom x 3) through the adjoint derivative kernels.
 int run_gradient(hfg_ctx* c, const double* D, double* grad);
+}  // namespace hfg
+
+namespace hfg {
+// Launch fn(class_work, stream) for every class on the side streams (fork/join with the main
+// stream): classes are independent until the gather.
+template <typename F>
+inline hipError_t for_each_class_concurrent(hfg_ctx* c, F&& fn) {
+    hipError_t e = hipEventRecord(c->ev_fork, c->stream);
+    if (e != hipSuccess) return e;
+    for (int k = 0; k < kSideStreams; ++k)
+        if ((e = hipStreamWaitEvent(c->side[k], c->ev_fork, 0)) != hipSuccess) return e;
+    for (size_t i = 0; i < c->classes.size(); ++i)
+        if ((e = fn(c->classes[i], c->side[c->class_stream[i]])) != hipSuccess) return e;
+    for (int k = 0; k < kSideStreams; ++k) {
+        if ((e = hipEventRecord(c->ev_join[k], c->side[k])) != hipSuccess) return e;
+        if ((e = hipStreamWaitEvent(c->stream, c->ev_join[k], 0)) != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
 }  // namespace hfg

@@ -33,11 +33,13 @@ int run_gradient(hfg_ctx* c, const double* D, double* grad) {
     a.prim_cut = c->opt.integral_tol / 1e4;
     a.slab = c->d_grad_slab.p;
     HFG_CHECK(c, hipEventRecord(c->ev0, c->stream));
-    for (auto& cw : c->classes) {
-        a.items = c->d_items.p + cw.dev_off;
-        a.item_base = (int)cw.dev_off;
-        HFG_CHECK(c, launch_grad(cw.la, cw.lb, cw.lc, cw.ld, c->pure, a, (int)cw.items.size(), c->stream));
-    }
+    HFG_CHECK(c, for_each_class_concurrent(c, [&](const ClassWork& cw, hipStream_t st) {
+        GradArgs b = a;
+        b.items = c->d_items.p + cw.dev_off;
+        b.item_base = (int)cw.dev_off;
+        return launch_grad(cw.la, cw.lb, cw.lc, cw.ld, c->pure, c->opt.precision == HFG_MIXED ? 1 : 0, b,
+                           (int)cw.items.size(), st);
+    }));
     HFG_CHECK(c, hipEventRecord(c->ev1, c->stream));
     HFG_CHECK(c, launch_grad_reduce(c->d_grad_slab.p, (int)c->total_items, nout, c->d_grad.p, c->stream));
     HFG_CHECK(c, hipMemcpyAsync(grad, c->d_grad.p, nout * sizeof(double), hipMemcpyDeviceToHost, c->stream));

@@ -1,16 +1,34 @@
-// One angular-momentum class of the adjoint gradient kernel; built once per class with
-// -DHFG_LA=.. -DHFG_LB=.. -DHFG_LC=.. -DHFG_LD=.. so the classes compile in parallel.
+// One angular-momentum class of the adjoint gradient kernel; built once per class and
+// precision with -DHFG_LA=.. -DHFG_LB=.. -DHFG_LC=.. -DHFG_LD=.. [-DHFG_MIXED=1] so the classes
+// compile in parallel.
 #include "grad_impl.hpp"
+
+#ifndef HFG_MIXED
+#define HFG_MIXED 0
+#endif
 
 namespace hfg {
 
+#if HFG_MIXED
+using Real = float;
+#define HFG_LAUNCH launch_grad_class32
+#else
+using Real = double;
+#define HFG_LAUNCH launch_grad_class64
+#endif
+
 template <>
-hipError_t launch_grad_class<HFG_LA, HFG_LB, HFG_LC, HFG_LD>(bool pure, const GradArgs& a, int nitems, hipStream_t s) {
-    const dim3 grid(nitems), block(256);
-    if (pure)
-        hipLaunchKernelGGL((grad_kernel<HFG_LA, HFG_LB, HFG_LC, HFG_LD, true>), grid, block, 0, s, a);
-    else
-        hipLaunchKernelGGL((grad_kernel<HFG_LA, HFG_LB, HFG_LC, HFG_LD, false>), grid, block, 0, s, a);
+hipError_t HFG_LAUNCH<HFG_LA, HFG_LB, HFG_LC, HFG_LD>(bool pure, const GradArgs& a, int nitems, hipStream_t s) {
+    GradArgs b = a;
+    b.nitems = nitems;
+    const int block = 256;
+    if (pure) {
+        auto k = grad_kernel<HFG_LA, HFG_LB, HFG_LC, HFG_LD, true, Real>;
+        hipLaunchKernelGGL(k, dim3(grid_for(k, nitems, block)), dim3(block), 0, s, b);
+    } else {
+        auto k = grad_kernel<HFG_LA, HFG_LB, HFG_LC, HFG_LD, false, Real>;
+        hipLaunchKernelGGL(k, dim3(grid_for(k, nitems, block)), dim3(block), 0, s, b);
+    }
     return hipGetLastError();
 }
 

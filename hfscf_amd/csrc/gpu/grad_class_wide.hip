@@ -1,0 +1,16 @@
+// The mixed-precision gradient launcher of the (dd|dd) class runs the FP64 kernel: its FP32
+// instantiation did not finish compiling within an hour (hipcc for gfx950), so this class keeps
+// FP64 recursions in the mixed mode. Every other class has its FP32 kernel.
+#include "grad_impl.hpp"
+
+namespace hfg {
+
+template <>
+hipError_t launch_grad_class64<2, 2, 2, 2>(bool, const GradArgs&, int, hipStream_t);
+
+template <>
+hipError_t launch_grad_class32<2, 2, 2, 2>(bool pure, const GradArgs& a, int nitems, hipStream_t s) {
+    return launch_grad_class64<2, 2, 2, 2>(pure, a, nitems, s);
+}
+
+}  // namespace hfg

@@ -3,16 +3,21 @@
 
 namespace hfg {
 
-#define HFG_DECL(A, B, C, D) \
-    template <>              \
-    hipError_t launch_grad_class<A, B, C, D>(bool, const GradArgs&, int, hipStream_t);
+#define HFG_DECL(A, B, C, D)                                                                 \
+    template <>                                                                              \
+    hipError_t launch_grad_class64<A, B, C, D>(bool, const GradArgs&, int, hipStream_t);     \
+    template <>                                                                              \
+    hipError_t launch_grad_class32<A, B, C, D>(bool, const GradArgs&, int, hipStream_t);
 HFG_FOR_CLASSES(HFG_DECL)
 #undef HFG_DECL
 
-hipError_t launch_grad(int la, int lb, int lc, int ld, bool pure, const GradArgs& a, int nitems, hipStream_t s) {
+hipError_t launch_grad(int la, int lb, int lc, int ld, bool pure, int mixed, const GradArgs& a, int nitems,
+                       hipStream_t s) {
     if (nitems == 0) return hipSuccess;
-#define HFG_CASE(A, B, C, D) \
-    if (la == A && lb == B && lc == C && ld == D) return launch_grad_class<A, B, C, D>(pure, a, nitems, s);
+#define HFG_CASE(A, B, C, D)                                                                 \
+    if (la == A && lb == B && lc == C && ld == D)                                            \
+        return mixed ? launch_grad_class32<A, B, C, D>(pure, a, nitems, s)                   \
+                     : launch_grad_class64<A, B, C, D>(pure, a, nitems, s);
     HFG_FOR_CLASSES(HFG_CASE)
 #undef HFG_CASE
     return hipErrorInvalidValue;

@@ -18,6 +18,7 @@ constexpr int kMaxAtoms = 128;
 
 struct GradArgs {
     const WorkItem* items;
+    int nitems;
     const PairDev* pairs;
     const double* psmean;
     const int* bp_off;
@@ -138,9 +139,10 @@ HFG_INLINE void hrr_from_vrr(const Vrr<LAB1, LCD1, R>& V, const R* AB, const R* 
     H::run(acc, AB, CD, out);
 }
 
-template <int LA, int LB, int LC, int LD, bool PURE>
+// R = double: FP64 throughout. R = float (mixed mode): two-particle density, VRR, HRR and
+// contraction in FP32; Boys values and pair data from FP64, per-item atom sums in FP64.
+template <int LA, int LB, int LC, int LD, bool PURE, typename R>
 __global__ __launch_bounds__(256) void grad_kernel(GradArgs a) {
-    using R = double;
     constexpr int NA = PURE && LA > 0 ? 2 * LA + 1 : ncart(LA);
     constexpr int NB = PURE && LB > 0 ? 2 * LB + 1 : ncart(LB);
     constexpr int NC = PURE && LC > 0 ? 2 * LC + 1 : ncart(LC);
@@ -149,7 +151,9 @@ __global__ __launch_bounds__(256) void grad_kernel(GradArgs a) {
     constexpr int LT = LA + LB + LC + LD;
     __shared__ double gsum[kMaxAtoms * 3];
     __shared__ int rowpre[kMaxRows + 1];
-    const WorkItem w = a.items[blockIdx.x];
+    for (int item = blockIdx.x; item < a.nitems; item += gridDim.x) {
+    const WorkItem w = a.items[item];
+    __syncthreads();  // the previous item's flush has read gsum
     for (int k = threadIdx.x; k < a.natom * 3; k += blockDim.x) gsum[k] = 0.0;
     const int nrows = w.r1 - w.r0;
     const int bpo = a.bp_off[w.P], kpo = a.bp_off[w.Q], kn = a.bp_cnt[w.Q];
@@ -158,7 +162,7 @@ __global__ __launch_bounds__(256) void grad_kernel(GradArgs a) {
         const int i = w.r0 + r;
         int c = ket_count(a.psmean + kpo, kn, a.psmean[bpo + i], a.tol);
         if (same) c = min(c, i + 1);
-        rowpre[r + 1] = c;
+        rowpre[r + 1] = max(0, min(c, w.c1) - w.c0);
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -177,7 +181,7 @@ __global__ __launch_bounds__(256) void grad_kernel(GradArgs a) {
                 hi = mid;
         }
         const int i = w.r0 + lo;
-        const int j = t - rowpre[lo];
+        const int j = w.c0 + t - rowpre[lo];
         const PairDev bp = a.pairs[bpo + i];
         const PairDev kp = a.pairs[kpo + j];
         double deg = 8.0;
@@ -201,7 +205,7 @@ __global__ __launch_bounds__(256) void grad_kernel(GradArgs a) {
                             const R dqs = a.D[(size_t)(ob + q) * n + od + s];
                             const R dps = a.D[(size_t)(oa + p) * n + od + s];
                             Gs[((p * NB + q) * NC + r) * ND + s] =
-                                (R)deg * (2.0 * dpq * drs - 0.5 * (dpr * dqs + dps * dqr));
+                                (R)deg * ((R)2 * dpq * drs - (R)0.5 * (dpr * dqs + dps * dqr));
                         });
                     });
                 });
@@ -209,8 +213,8 @@ __global__ __launch_bounds__(256) void grad_kernel(GradArgs a) {
             SphT<LA, LB, LC, LD, PURE, R>::run(Gs, G);
         }
         R gA[3] = {0, 0, 0}, gB[3] = {0, 0, 0}, gC[3] = {0, 0, 0};
-        const R AB[3] = {bp.AB[0], bp.AB[1], bp.AB[2]};
-        const R CD[3] = {kp.AB[0], kp.AB[1], kp.AB[2]};
+        const R AB[3] = {(R)bp.AB[0], (R)bp.AB[1], (R)bp.AB[2]};
+        const R CD[3] = {(R)kp.AB[0], (R)kp.AB[1], (R)kp.AB[2]};
         for (int ip = 0; ip < bp.np; ++ip) {
             const double4 a0 = a.pp0[bp.off + ip];
             const double4 a1 = a.pp1[bp.off + ip];
@@ -231,18 +235,19 @@ __global__ __launch_bounds__(256) void grad_kernel(GradArgs a) {
                 R F[LT + 2];
                 boys_eval<LT + 1, R>(T, a.boys, pfac, F);
                 const double rp = q * ipq, rq = p * ipq;
-                const R PA[3] = {a1.x, a1.y, a1.z};
-                const R QC[3] = {b1.x, b1.y, b1.z};
-                const R WP[3] = {-rp * PQx, -rp * PQy, -rp * PQz};
-                const R WQ[3] = {rq * PQx, rq * PQy, rq * PQz};
+                const R PA[3] = {(R)a1.x, (R)a1.y, (R)a1.z};
+                const R QC[3] = {(R)b1.x, (R)b1.y, (R)b1.z};
+                const R WP[3] = {(R)(-rp * PQx), (R)(-rp * PQy), (R)(-rp * PQz)};
+                const R WQ[3] = {(R)(rq * PQx), (R)(rq * PQy), (R)(rq * PQz)};
                 Vrr<LA + LB + 1, LC + LD + 1, R> V;
-                V.run(F, PA, WP, QC, WQ, a2.x, rp, b2.x, rq, 0.5 * ipq);
+                V.run(F, PA, WP, QC, WQ, (R)a2.x, (R)rp, (R)b2.x, (R)rq, (R)(0.5 * ipq));
+                const R wa = (R)(2.0 * a2.y), wb = (R)(2.0 * a2.z), wc = (R)(2.0 * b2.y);
                 R s[3];
                 {  // centre A
                     R I[ncart(LA + 1) * cb * cc * cd];
                     hrr_from_vrr<LA + 1, LB, LC, LD>(V, AB, CD, I);
                     contract_shift<LA, LB, LC, LD, 0, 1, R>(G, I, s);
-                    sfor<3>([&](auto d) { gA[d] += 2.0 * a2.y * s[d]; });
+                    sfor<3>([&](auto d) { gA[d] += wa * s[d]; });
                     if constexpr (LA > 0) {
                         R Im[ncart(LA - 1) * cb * cc * cd];
                         hrr_from_vrr<LA - 1, LB, LC, LD>(V, AB, CD, Im);
@@ -254,7 +259,7 @@ __global__ __launch_bounds__(256) void grad_kernel(GradArgs a) {
                     R I[ca * ncart(LB + 1) * cc * cd];
                     hrr_from_vrr<LA, LB + 1, LC, LD>(V, AB, CD, I);
                     contract_shift<LA, LB, LC, LD, 1, 1, R>(G, I, s);
-                    sfor<3>([&](auto d) { gB[d] += 2.0 * a2.z * s[d]; });
+                    sfor<3>([&](auto d) { gB[d] += wb * s[d]; });
                     if constexpr (LB > 0) {
                         R Im[ca * ncart(LB - 1) * cc * cd];
                         hrr_from_vrr<LA, LB - 1, LC, LD>(V, AB, CD, Im);
@@ -266,7 +271,7 @@ __global__ __launch_bounds__(256) void grad_kernel(GradArgs a) {
                     R I[ca * cb * ncart(LC + 1) * cd];
                     hrr_from_vrr<LA, LB, LC + 1, LD>(V, AB, CD, I);
                     contract_shift<LA, LB, LC, LD, 2, 1, R>(G, I, s);
-                    sfor<3>([&](auto d) { gC[d] += 2.0 * b2.y * s[d]; });
+                    sfor<3>([&](auto d) { gC[d] += wc * s[d]; });
                     if constexpr (LC > 0) {
                         R Im[ca * cb * ncart(LC - 1) * cd];
                         hrr_from_vrr<LA, LB, LC - 1, LD>(V, AB, CD, Im);
@@ -279,21 +284,26 @@ __global__ __launch_bounds__(256) void grad_kernel(GradArgs a) {
         const int atA = a.shell_atom[bp.sa], atB = a.shell_atom[bp.sb];
         const int atC = a.shell_atom[kp.sa], atD = a.shell_atom[kp.sb];
         for (int d = 0; d < 3; ++d) {
-            atomicAdd(&gsum[atA * 3 + d], gA[d]);
-            atomicAdd(&gsum[atB * 3 + d], gB[d]);
-            atomicAdd(&gsum[atC * 3 + d], gC[d]);
-            atomicAdd(&gsum[atD * 3 + d], -(gA[d] + gB[d] + gC[d]));
+            const double xa = gA[d], xb = gB[d], xc = gC[d];
+            atomicAdd(&gsum[atA * 3 + d], xa);
+            atomicAdd(&gsum[atB * 3 + d], xb);
+            atomicAdd(&gsum[atC * 3 + d], xc);
+            atomicAdd(&gsum[atD * 3 + d], -(xa + xb + xc));
         }
     }
     __syncthreads();
-    double* out = a.slab + (size_t)(a.item_base + blockIdx.x) * a.natom * 3;
+    double* out = a.slab + (size_t)(a.item_base + item) * a.natom * 3;
     for (int k = threadIdx.x; k < a.natom * 3; k += blockDim.x) out[k] = gsum[k];
+    }
 }
 
 template <int LA, int LB, int LC, int LD>
-hipError_t launch_grad_class(bool pure, const GradArgs& a, int nitems, hipStream_t s);
+hipError_t launch_grad_class64(bool pure, const GradArgs& a, int nitems, hipStream_t s);
+template <int LA, int LB, int LC, int LD>
+hipError_t launch_grad_class32(bool pure, const GradArgs& a, int nitems, hipStream_t s);
 
-hipError_t launch_grad(int la, int lb, int lc, int ld, bool pure, const GradArgs& a, int nitems, hipStream_t s);
+hipError_t launch_grad(int la, int lb, int lc, int ld, bool pure, int mixed, const GradArgs& a, int nitems,
+                       hipStream_t s);
 hipError_t launch_grad_reduce(const double* slab, int nitems, int nout, double* out, hipStream_t s);
 
 }  // namespace hfg

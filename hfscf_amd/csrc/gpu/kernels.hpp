@@ -8,6 +8,7 @@ namespace hfg {
 
 struct JkArgs {
     const WorkItem* items;
+    int nitems;
     const PairDev* pairs;
     const double* psmean;  // mean Schwarz value per pair (parallel to pairs)
     const int* bp_off;     // block pair -> first pair
@@ -24,6 +25,12 @@ struct JkArgs {
     double tol;        // shell-quartet Schwarz threshold on mean values
     double prim_cut;   // primitive-quartet prefactor cutoff
     double* slab;      // FP64 tiles per work item
+    // density-weighted screening (incremental builds): a quartet is skipped when
+    // sqrt(ab|ab) sqrt(cd|cd) max(4|D_ab|, 4|D_cd|, |D_ac|, |D_ad|, |D_bc|, |D_bd|) < dtol,
+    // with |D_xy| the largest density element of shell block (x, y); dtol = 0 disables it
+    const float* dmax;  // nshell x nshell
+    int nshell;
+    double dtol;
 };
 
 struct GatherArgs {
@@ -54,7 +61,8 @@ struct ScreenArgs {  // Cartesian (ab|ab) diagonal per pair
     double* screen;  // ncart x ncart
 };
 
-// Launchers (one specialised kernel per class; la >= lb, la >= lc >= ld).
+// Launchers (one specialised kernel per class; la >= lb, la >= lc >= ld). mixed != 0 selects
+// the FP32 evaluation and digestion of the class (FP64 Boys values, tiles and sums).
 hipError_t launch_jk(int la, int lb, int lc, int ld, bool pure, int mixed, const JkArgs& a, int nitems, hipStream_t s);
 hipError_t launch_screen(int la, int lb, const ScreenArgs& a, hipStream_t s);
 hipError_t launch_gather(const GatherArgs& a, int ntiles, hipStream_t s);

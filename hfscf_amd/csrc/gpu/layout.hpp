@@ -24,6 +24,7 @@ struct PairDev {
     int ao_a, ao_b;    // first output-basis function of sa / sb
     int sq_off;        // offset of sqrt|(pq|pq)| per Cartesian component pair (p in sa, q in sb)
     double sq_min;     // smallest of those values
+    double sq_max;     // largest of those values (density-weighted screening bound)
     double AB[3];      // A - B
 };
 
@@ -36,9 +37,14 @@ struct PairDev {
 struct WorkItem {
     int P, Q;          // bra / ket block pair
     int r0, r1;        // bra rows [r0, r1) of P's pair list
+    int c0, c1;        // ket columns [c0, c1) of each row (clipped to the row's surviving range)
     int A, B, C, D;    // blocks
     long long slab;    // offset (doubles) of this item's six tiles
 };
+
+// Largest scratch (private segment) one dispatch may need; launches of spill-heavy class
+// kernels are given fewer workgroups (each loops over several work items) to stay below it.
+constexpr double kScratchBudget = 1024.0 * 1024.0 * 1024.0;
 
 // Gather list entry for one output tile: a tile of `rows x cols` at `off` (row-major),
 // read transposed when trans != 0.

@@ -30,12 +30,13 @@ int set_err(const std::string& s) {
 // GPU backend: ERIOS/hf_solver/scf_gradient two-electron work through the hfgpu C ABI.
 class GpuJK final : public JKBuilder {
    public:
-    GpuJK(const Molecule& mol, const hfscf_scf_options& o) : mol_(mol) {
+    GpuJK(const Molecule& mol, const hfscf_scf_options& o) : mol_(mol), precision_(o.precision) {
         hfg_options go;
         hfg_default_options(&go);
         go.precision = o.precision;
         go.integral_tol = o.integral_tol;
         go.schwarz = o.schwarz;
+        go.density_tol = o.density_tol;
         int r = hfg_create(&ctx_, o.device, &go);
         if (r) throw std::runtime_error("hfg_create failed (" + std::to_string(r) + "): no usable GPU");
         const int ns = (int)mol.shells.size();
@@ -60,13 +61,20 @@ class GpuJK final : public JKBuilder {
     ~GpuJK() override { hfg_destroy(ctx_); }
     void build_jk(const double* D, double* J, double* K) override { check(hfg_build_jk(ctx_, D, J, K, 0)); }
     void fock(const double* H, const double* D, int incremental, double* F) override {
+        // mixed precision: FP32 only for builds on a density difference, whose FP32 error
+        // shrinks with the difference; full builds (the first, the final one) in FP64
+        const bool inc = incremental && have_prev_;
+        check(hfg_set_precision(ctx_, (precision_ == HFG_MIXED && inc && !final_) ? HFG_MIXED : HFG_FP64));
         check(hfg_fock(ctx_, H, D, incremental, F));
+        have_prev_ = true;
     }
     void gradient_2e(const double* D, double* grad) override {
         std::vector<double> g((size_t)mol_.natom() * 3);
+        check(hfg_set_precision(ctx_, precision_));
         check(hfg_energy_backward(ctx_, D, g.data()));
         for (size_t k = 0; k < g.size(); ++k) grad[k] += g[k];
     }
+    void set_final(bool final_build) override { final_ = final_build; }
 
    private:
     void check(int r) {
@@ -74,6 +82,8 @@ class GpuJK final : public JKBuilder {
     }
     const Molecule& mol_;
     hfg_ctx* ctx_ = nullptr;
+    int precision_;
+    bool have_prev_ = false, final_ = false;
 };
 
 // Callback backend (an external Fock builder behind the same seam).
@@ -127,6 +137,7 @@ void hfscf_default_scf_options(hfscf_scf_options* o) {
     o->rms_tol = 1e-10;
     o->integral_tol = 1e-14;
     o->final_fp64 = 1;
+    o->density_tol = 1e-11;
 }
 
 hfscf_mol* hfscf_mol_create(int natom, const int* Z, const double* xyz_bohr, const char* gbs_path, int pure_override) {
@@ -254,6 +265,7 @@ int hfscf_rhf_run(hfscf_mol* m, const hfscf_scf_options* o, const hfscf_callback
         so.rms_tol = opt.rms_tol;
         so.max_iter = opt.max_iter;
         so.incremental = opt.incremental != 0;
+        so.final_build = opt.final_fp64 != 0;
         so.verbose = opt.verbose;
         hf_solver solver(m->mol, so, jk.get());
         solver.init_data();
@@ -267,6 +279,7 @@ int hfscf_rhf_run(hfscf_mol* m, const hfscf_scf_options* o, const hfscf_callback
             res->fock_seconds = solver.total_fock_seconds();
             res->last_fock_seconds = solver.last_fock_seconds();
             res->fock_builds = solver.iterations();
+            res->energy_final = solver.energy_final_total();
         }
         const size_t nn = (size_t)m->mol.nbf() * m->mol.nbf();
         if (D) std::memcpy(D, solver.density().data(), nn * sizeof(double));

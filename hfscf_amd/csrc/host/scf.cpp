@@ -170,6 +170,17 @@ bool hf_solver::scf_run() {
         }
         if (iteration_ >= opt_.max_iter) break;
     }
+    if (opt_.final_build) {
+        jk_->set_final(true);
+        std::vector<double> Ff((size_t)n_ * n_);
+        jk_->fock(H_.data(), D_.data(), 0, Ff.data());
+        double e = 0.0;
+        for (size_t k = 0; k < Ff.size(); ++k) e += D_[k] * (H_[k] + Ff[k]);
+        e_final_ = e;
+        jk_->set_final(false);
+    } else {
+        e_final_ = e_cur_;
+    }
     // final coefficients from the last orthogonalised Fock matrix (hfscf_hf_solver.cpp:469-473)
     std::vector<double> U((size_t)n_ * n_);
     eps_.resize(n_);

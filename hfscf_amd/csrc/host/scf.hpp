@@ -35,6 +35,7 @@ struct ScfOptions {
     int max_iter = 100;
     int charge = 0;
     bool incremental = false;  // incremental (delta-D) Fock builds
+    bool final_build = true;   // full-precision Fock build at the converged density
     int verbose = 0;
 };
 
@@ -60,6 +61,9 @@ class hf_solver {
     bool scf_run();    // hfscf: src/scfsolver/hfscf_hf_solver.cpp:389-476
 
     double energy_electronic() const { return e_cur_; }
+    // E = sum D (H + F(D)) from one more full-precision Fock build at the converged density
+    // (stationary in D, so free of the DIIS-extrapolation noise in hfscf's energy expression)
+    double energy_final_total() const { return e_final_ + mol_.nuclear_repulsion(); }
     double energy_total() const { return e_cur_ + mol_.nuclear_repulsion(); }
     int iterations() const { return iteration_; }
     int nbf() const { return n_; }
@@ -82,7 +86,7 @@ class hf_solver {
     JKBuilder* jk_;
     int n_ = 0, nocc_ = 0, iteration_ = 0;
     std::vector<double> S_, T_, V_, H_, X_, F_, Fp_, D_, C_, eps_, energies_;
-    double e_cur_ = 0.0, e_prev_ = 0.0;
+    double e_cur_ = 0.0, e_prev_ = 0.0, e_final_ = 0.0;
     double fock_seconds_ = 0.0, fock_total_ = 0.0;
     std::unique_ptr<diis_solver> diis_;
 };

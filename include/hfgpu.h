@@ -47,10 +47,14 @@ typedef struct hfg_options {
     double integral_tol;  /* hfscf integral_tol (default 1e-14): Schwarz threshold is 100x this,
                              primitive cutoff integral_tol/1e4 (hfscf_oseri.hpp:31-32, hfscf_oseri.cpp:73) */
     int schwarz;          /* 1 = screen shell quartets on the mean Schwarz bound (hfscf eri_screen) */
-    int reserved[8];
+    double density_tol;   /* incremental builds only: skip a quartet when its density-weighted Schwarz
+                             bound sqrt(ab|ab) sqrt(cd|cd) max(4|dD_ab|, 4|dD_cd|, |dD_ac|, |dD_ad|,
+                             |dD_bc|, |dD_bd|) is below this (the bound of hfscf's direct digest,
+                             hfscf_hf_solver.cpp:317-324); 0 = off (default) */
+    int reserved[6];
 } hfg_options;
 
-/* Fill opt with defaults (FP64, integral_tol 1e-14, screening on). */
+/* Fill opt with defaults (FP64, integral_tol 1e-14, screening on, no density screening). */
 void hfg_default_options(hfg_options* opt);
 
 /* Create a context on HIP device `device`. Replaces: ERIOS::Erios::Erios(is_pure, cut_off). */
@@ -81,6 +85,11 @@ int hfg_fock(hfg_ctx* ctx, const double* H, const double* D, int incremental, do
 
 /* Forget the previous density (next hfg_fock is a full build). */
 int hfg_reset_incremental(hfg_ctx* ctx);
+
+/* Precision of the following builds and gradients (hfg_precision). A mixed-precision SCF runs
+ * its first (full) build in FP64, the incremental builds in HFG_MIXED and a final FP64 build
+ * on the converged density; the previous Fock matrix is kept across the switch. */
+int hfg_set_precision(hfg_ctx* ctx, int precision);
 
 /* E_2e = sum_ij D_ij (2J - K)_ij for the half density D. */
 int hfg_energy(hfg_ctx* ctx, const double* D, double* E2);

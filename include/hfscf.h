@@ -33,7 +33,8 @@ typedef struct hfscf_callbacks {
 typedef struct hfscf_scf_options {
     int backend;          /* HFSCF_BACKEND_* */
     int device;           /* HIP device for the GPU backend */
-    int precision;        /* hfg_precision */
+    int precision;        /* hfg_precision; HFG_MIXED: first build FP64, incremental builds FP32,
+                             final build FP64 */
     int incremental;      /* delta-D Fock builds */
     int diis_range;       /* hfscf diis_size (vectors kept = range + 1) */
     int max_iter;
@@ -42,8 +43,9 @@ typedef struct hfscf_scf_options {
     double energy_tol;
     double rms_tol;
     double integral_tol;
-    int final_fp64;       /* mixed precision: finish with FP64 builds until converged again */
-    int reserved[7];
+    int final_fp64;       /* one more full FP64 Fock build at the converged density (energy_final) */
+    double density_tol;   /* density-weighted quartet screening of incremental builds (hfg_options) */
+    int reserved[5];
 } hfscf_scf_options;
 
 typedef struct hfscf_scf_result {
@@ -55,6 +57,7 @@ typedef struct hfscf_scf_result {
     double fock_seconds;  /* wall time inside Fock builds */
     double last_fock_seconds;
     int fock_builds;
+    double energy_final;  /* sum D (H + F(D)) + E_nuc from a full-precision build at the final D */
 } hfscf_scf_result;
 
 void hfscf_default_scf_options(hfscf_scf_options* o);

@@ -57,3 +57,30 @@ def test_config_sizes():
              ("h2o.xyz", "sto-3g"): 7}
     for (xyz, basis), n in sizes.items():
         assert H.Molecule.from_xyz(os.path.join(ROOT, "tests", "golden", xyz), basis).nbf == n
+
+
+def test_struct_layouts_match_headers(tmp_path):
+    # ctypes mirrors of the option/result/stats structs against the C compiler's layout
+    import ctypes as C
+    import subprocess
+    structs = {"hfg_options": H.HfgOptions, "hfg_stats": H.HfgStats, "hfscf_scf_options": H.ScfOptions,
+               "hfscf_scf_result": H.ScfResult}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "hfgpu.h"', '#include "hfscf.h"',
+             "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in py._fields_:
+            lines.append(f'printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = {}
+    for line in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.splitlines():
+        cname, fname, v = line.split()
+        got[(cname, fname)] = int(v)
+    for cname, py in structs.items():
+        assert got[(cname, "size")] == C.sizeof(py), cname
+        for fname, _ in py._fields_:
+            assert got[(cname, fname)] == getattr(py, fname).offset, (cname, fname)

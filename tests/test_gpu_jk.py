@@ -142,7 +142,11 @@ def test_cfg2_h2o8_ccpvdz_fixture():
     print(f"cfg2 J/K rows: |dJ| {dj:.2e} |dK| {dk:.2e}; sum D.J {np.sum(D * J) - float(f['EJ']):.2e}", flush=True)
     assert dj < 1e-11 and dk < 1e-11
     scf = H.rhf(mol, diis_range=8)
-    print(f"cfg2 SCF: E {scf['energy']:.12f} dE vs oracle {scf['energy'] - float(f['energy']):.2e} "
-          f"iterations {scf['iterations']}", flush=True)
+    # the energy of one more full build at the converged density (stationary in D) is compared;
+    # hfscf's loop energy uses the DIIS-extrapolated Fock matrix and carries the convergence noise
+    key = "energy_final" if "energy_final" in f else "energy"
+    de = scf[key] - float(f[key])
+    print(f"cfg2 SCF: E {scf[key]:.12f} dE vs oracle {de:.2e} (loop energy "
+          f"{scf['energy'] - float(f['energy']):.2e}) iterations {scf['iterations']}", flush=True)
     assert scf["converged"]
-    assert abs(scf["energy"] - float(f["energy"])) < 1e-9
+    assert abs(de) < 1e-9
