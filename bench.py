@@ -5,14 +5,15 @@ digested and summed on the GPU) for the converged density of the workload. Defau
 (H2O)8 / cc-pVDZ (pure d, 192 basis functions), Schwarz threshold 1e-12, FP64.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload h2o8|c12h26|h2o16]
-                    [--no-cpu] [--no-extra] [--no-mixed]
+                    [--no-cpu] [--no-extra] [--no-mixed] [--no-scf] [--batch-molecules N]
 
 Prints one JSON line (value = FP64 Fock builds/s on the GPU) with a roofline object for the
 J/K kernels (model FLOPs of the surviving quartets / HIP-event kernel time, against the FP64
 vector peak) and the CPU baseline (the oracle, timed on a bounded sample on this host).
 "extra" holds the other figures of the metric: two-electron gradients/s (adjoint kernel),
 the mixed-precision Fock build and gradient rates at the same density with their errors
-against FP64, and whole-SCF timings of the FP64 and mixed (incremental) solvers with |dE|.
+against FP64, whole-SCF timings of the FP64 and mixed (incremental) solvers with |dE|, and the
+batching mode (512 seeded water clusters in one lockstep SCF) against one-at-a-time solves.
 """
 import argparse
 import json
@@ -115,44 +116,107 @@ def cpu_baseline(mol, budget_rows=32):
     }
 
 
-def extra_figures(mol, Hc, D, ref_F, ref_g, scf64, steps, warmup, mixed):
-    out = {}
+def dominant_roofline(classes):
+    """Roofline of the dominant J/K kernel (the class launch with the longest duration in the last
+    build): model FLOPs of its surviving quartets / its HIP-event duration, against FP64 vector peak.
+    HBM traffic per launch comes from the profiler's counters (profiles/), not from this run."""
+    c = max(classes, key=lambda c: c["kernel_ms"])
+    ach = c["model_flops"] / (c["kernel_ms"] * 1e-3) / 1e12
+    return {"bound": "valu", "kernel": f"jk_kernel ({c['cls'][0]}{c['cls'][1]}|{c['cls'][2]}{c['cls'][3]}) fp64",
+            "achieved": ach, "peak": PEAK_FP64, "unit": "TFLOP/s", "frac": ach / PEAK_FP64, "traffic": None,
+            "model_gflop_per_launch": c["model_flops"] / 1e9, "kernel_ms": c["kernel_ms"]}
+
+
+def gradient_figures(mol, D, steps, warmup):
+    """FP64 two-electron gradients/s (adjoint kernel) at the workload's density."""
     ctx = H.GpuContext(mol)
-    el, kern, g64 = time_grads(ctx, D, steps, warmup)
+    el, kern, g = time_grads(ctx, D, steps, warmup)
     st = ctx.stats()
-    out["fp64_gradients_per_s"] = steps / el
-    out["fp64_gradient_kernel_ms"] = 1e3 * kern / steps
-    out["fp64_gradient_model_tflops"] = st["model_flops"] / (kern / steps) / 1e12
-    log(f"FP64 gradient: {steps / el:.2f}/s, kernel {1e3 * kern / steps:.2f} ms")
     ctx.close()
-    if not mixed:
-        return out
-    ctxm = H.GpuContext(mol, precision=H.MIXED)
-    el, kern = time_builds(ctxm, Hc, D, steps, warmup)
-    Fm = ctxm.fock(Hc, D)
-    out["mixed_fock_builds_per_s"] = steps / el
-    out["mixed_fock_kernel_ms"] = 1e3 * kern / steps
-    out["mixed_fock_max_abs_err_vs_fp64"] = float(np.abs(Fm - ref_F).max())
-    out["mixed_energy_err_at_fixed_D"] = float(np.sum(D * (Fm - ref_F)))
-    el, kern, gm = time_grads(ctxm, D, steps, warmup)
-    out["mixed_gradients_per_s"] = steps / el
-    out["mixed_gradient_kernel_ms"] = 1e3 * kern / steps
-    out["mixed_gradient_max_abs_err_vs_fp64"] = float(np.abs(gm - g64).max())
-    log(f"mixed build {out['mixed_fock_builds_per_s']:.2f}/s, gradient {out['mixed_gradients_per_s']:.2f}/s")
-    ctxm.close()
-    # whole SCF: FP64 (full builds, as hfscf) vs mixed (FP32 incremental builds + final FP64)
+    log(f"FP64 gradient: {steps / el:.2f}/s, kernel {1e3 * kern / steps:.2f} ms")
+    return {"fp64_gradients_per_s": steps / el, "fp64_gradient_kernel_ms": 1e3 * kern / steps,
+            "fp64_gradient_model_tflops": st["model_flops"] / (kern / steps) / 1e12}
+
+
+def mixed_figures(workload, steps, warmup):
+    """Mixed precision against FP64 on one workload ((H2O)16 / cc-pVDZ by default): Fock builds/s
+    and gradients/s at the converged FP64 density (full builds, no density difference), and whole
+    SCF runs, FP64 with full builds as hfscf does and mixed with FP32 density-difference builds,
+    density screening and a final FP64 build, with |dE| and the gradient error."""
+    xyz, basis, desc = WORKLOADS[workload]
+    mol = H.Molecule.from_xyz(os.path.join(ROOT, "tests", "golden", xyz), basis)
+    S, T, V = mol.oneint()
+    Hc = T + V
+    out = {"mixed_workload": desc}
+    t0 = time.perf_counter()
+    scf64 = H.rhf(mol, **SCF_OPTS)
+    out["scf_fp64_seconds"] = time.perf_counter() - t0
     t0 = time.perf_counter()
     scfm = H.rhf(mol, precision=H.MIXED, incremental=1, **SCF_OPTS)
-    tm = time.perf_counter() - t0
-    out["scf_fp64_seconds"] = scf64["seconds"]
+    out["scf_mixed_seconds"] = time.perf_counter() - t0
     out["scf_fp64_fock_seconds"] = scf64["fock_seconds"]
-    out["scf_fp64_iterations"] = scf64["iterations"]
-    out["scf_mixed_seconds"] = tm
     out["scf_mixed_fock_seconds"] = scfm["fock_seconds"]
+    out["scf_fp64_iterations"] = scf64["iterations"]
     out["scf_mixed_iterations"] = scfm["iterations"]
     out["scf_mixed_dE_vs_fp64"] = scfm["energy_final"] - scf64["energy_final"]
-    log(f"SCF mixed: {tm:.2f} s (Fock {scfm['fock_seconds']:.2f} s), dE {out['scf_mixed_dE_vs_fp64']:.2e}")
+    out["scf_fock_speedup_mixed_vs_fp64"] = scf64["fock_seconds"] / max(scfm["fock_seconds"], 1e-12)
+    log(f"{workload} SCF FP64 {out['scf_fp64_seconds']:.2f} s (Fock {scf64['fock_seconds']:.2f} s), mixed "
+        f"{out['scf_mixed_seconds']:.2f} s (Fock {scfm['fock_seconds']:.2f} s), dE {out['scf_mixed_dE_vs_fp64']:.2e}")
+    D = scf64["D"]
+    res = {}
+    for tag, prec in [("fp64", H.FP64), ("mixed", H.MIXED)]:
+        ctx = H.GpuContext(mol, precision=prec)
+        el, kern = time_builds(ctx, Hc, D, steps, warmup)
+        F = ctx.fock(Hc, D)
+        elg, kerng, g = time_grads(ctx, D, steps, warmup)
+        ctx.close()
+        res[tag] = (F, g)
+        out[f"{tag}_fock_builds_per_s"] = steps / el
+        out[f"{tag}_fock_kernel_ms"] = 1e3 * kern / steps
+        out[f"{tag}_gradients_per_s"] = steps / elg
+        out[f"{tag}_gradient_kernel_ms"] = 1e3 * kerng / steps
+    out["mixed_fock_max_abs_err_vs_fp64"] = float(np.abs(res["mixed"][0] - res["fp64"][0]).max())
+    out["mixed_energy_err_at_fixed_D"] = float(np.sum(D * (res["mixed"][0] - res["fp64"][0])))
+    out["mixed_gradient_max_abs_err_vs_fp64"] = float(np.abs(res["mixed"][1] - res["fp64"][1]).max())
+    out["fock_speedup_mixed_vs_fp64"] = out["mixed_fock_builds_per_s"] / out["fp64_fock_builds_per_s"]
+    out["gradient_speedup_mixed_vs_fp64"] = out["mixed_gradients_per_s"] / out["fp64_gradients_per_s"]
+    log(f"{workload} builds/s FP64 {out['fp64_fock_builds_per_s']:.2f} mixed {out['mixed_fock_builds_per_s']:.2f}; "
+        f"gradients/s FP64 {out['fp64_gradients_per_s']:.2f} mixed {out['mixed_gradients_per_s']:.2f}")
     return out
+
+
+def batch_figures(count, sample, seed=11):
+    """Batching mode on the seeded (H2O)n clusters (n in 1..10, 6-31G*): all molecules in one
+    lockstep SCF with one batched build per iteration (HIP graphs, autotuned launch shapes), then
+    gradients in one batched launch; against one-at-a-time GPU solves of a random sample."""
+    frames = H.read_xyz_frames(os.path.join(ROOT, "tests", "golden", "batch_h2o.xyz"))[:count]
+    mols = [H.Molecule.from_arrays(Z, X, "6-31g*") for Z, X in frames]
+    opts = dict(diis_range=6, energy_tol=1e-10, rms_tol=1e-9, incremental=1)
+    t0 = time.perf_counter()
+    res, st = H.batch_rhf(mols, gradient=True, graph=1, autotune=1, **opts)
+    tb = time.perf_counter() - t0
+    log(f"batch of {len(mols)}: {tb:.2f} s ({len(mols) / tb:.1f} molecules/s), stats {st}")
+    rng = np.random.default_rng(seed)
+    idx = sorted(rng.choice(len(mols), size=min(sample, len(mols)), replace=False).tolist())
+    t_solo, de, dg = 0.0, 0.0, 0.0
+    for k in idx:
+        t0 = time.perf_counter()
+        r = H.rhf(mols[k], **opts)
+        g = H.rhf_gradient(mols[k], r["D"], r["F"])
+        t_solo += time.perf_counter() - t0
+        de = max(de, abs(r["energy_final"] - res[k]["energy_final"]))
+        dg = max(dg, float(np.abs(g - res[k]["gradient"]).max()))
+    solo_rate = len(idx) / t_solo
+    log(f"solo sample of {len(idx)}: {solo_rate:.2f} molecules/s, max |dE| {de:.2e}, max |dg| {dg:.2e}")
+    nconv = sum(r["converged"] for r in res)
+    return {"batch_molecules": len(mols), "batch_converged": nconv, "batch_seconds": tb,
+            "batch_molecules_per_s": len(mols) / tb, "batch_iterations": st["iterations"],
+            "batch_ms_per_iteration": 1e3 * st["scf_seconds"] / max(st["iterations"], 1),
+            "batch_setup_seconds": st["setup_seconds"], "batch_autotune_seconds": st["autotune_seconds"],
+            "batch_gradient_seconds": st["gradient_seconds"], "batch_quartets_per_build": st["quartets"],
+            "solo_sample": len(idx), "solo_molecules_per_s": solo_rate,
+            "batch_speedup_vs_solo": (len(mols) / tb) / solo_rate,
+            "batch_vs_solo_max_dE": de, "batch_vs_solo_max_dgrad": dg}
 
 
 def main():
@@ -169,6 +233,12 @@ def main():
                          "(FP64 build cost does not depend on the density values; for profiler runs)")
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="MIB",
                     help="also run a plain device copy of this many MiB (HBM counter calibration)")
+    ap.add_argument("--mixed-workload", default="h2o16", choices=sorted(WORKLOADS),
+                    help="workload of the mixed-precision against FP64 figures")
+    ap.add_argument("--batch-molecules", type=int, default=512,
+                    help="batching-mode figures over this many seeded water clusters (0 = skip)")
+    ap.add_argument("--solo-sample", type=int, default=24,
+                    help="molecules solved one at a time for the batching comparison")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (see DESIGN.md, sharding)",
@@ -196,7 +266,7 @@ def main():
     ctx = H.GpuContext(mol)
     el, kern = time_builds(ctx, Hc, D, args.steps, args.warmup)
     st = ctx.stats()
-    F64 = ctx.fock(Hc, D)
+    classes = ctx.class_stats()  # the last build's class launches, bracketed by HIP events
     ctx.close()
     builds_s = args.steps / el
     kern_per = kern / args.steps
@@ -219,9 +289,15 @@ def main():
                    "scf_energy": scf["energy"], "scf_energy_final": scf["energy_final"],
                    "scf_iterations": scf["iterations"], "scf_seconds": scf["seconds"],
                    "quartets_per_s": st["quartets"] * builds_s},
-        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP64, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64, "traffic": None,
-                     "model_gflop_per_build": st["model_flops"] / 1e9, "kernel_ms_per_build": 1e3 * kern_per},
+        "roofline": dominant_roofline(classes),
+        "build_roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP64, "unit": "TFLOP/s",
+                           "frac": achieved / PEAK_FP64, "model_gflop_per_build": st["model_flops"] / 1e9,
+                           "kernel_ms_per_build": 1e3 * kern_per,
+                           "note": "all class launches of one build (four streams) over the event span"},
+        "classes": [{"cls": c["cls"], "quartets": c["quartets"], "model_gflop": c["model_flops"] / 1e9,
+                     "kernel_ms": c["kernel_ms"],
+                     "tflops": c["model_flops"] / c["kernel_ms"] / 1e9 if c["kernel_ms"] > 0 else None}
+                    for c in sorted(classes, key=lambda c: -c["kernel_ms"])],
     }
     ref_path = os.path.join(ROOT, "tests", "golden", "cfg2_h2o8_ccpvdz.npz")
     if args.workload == "h2o8" and os.path.exists(ref_path):
@@ -230,7 +306,11 @@ def main():
         if "energy_final" in f:
             out["config"]["dE_final_vs_oracle"] = scf["energy_final"] - float(f["energy_final"])
     if not args.no_extra:
-        out["extra"] = extra_figures(mol, Hc, D, F64, None, scf, max(2, args.steps // 2), 1, not args.no_mixed)
+        out["extra"] = gradient_figures(mol, D, max(2, args.steps // 2), 1)
+        if not args.no_mixed:
+            out["extra"].update(mixed_figures(args.mixed_workload, max(2, args.steps // 2), 1))
+        if args.batch_molecules > 0:
+            out["extra"].update(batch_figures(args.batch_molecules, args.solo_sample))
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(mol)
     print(json.dumps(out), flush=True)

@@ -28,7 +28,7 @@ FP64, MIXED = 0, 1
 
 class HfgOptions(C.Structure):
     _fields_ = [("precision", C.c_int), ("integral_tol", C.c_double), ("schwarz", C.c_int),
-                ("density_tol", C.c_double), ("reserved", C.c_int * 6)]
+                ("density_tol", C.c_double), ("graph", C.c_int), ("reserved", C.c_int * 5)]
 
 
 class HfgStats(C.Structure):
@@ -40,13 +40,20 @@ class ScfOptions(C.Structure):
     _fields_ = [("backend", C.c_int), ("device", C.c_int), ("precision", C.c_int), ("incremental", C.c_int),
                 ("diis_range", C.c_int), ("max_iter", C.c_int), ("schwarz", C.c_int), ("verbose", C.c_int),
                 ("energy_tol", C.c_double), ("rms_tol", C.c_double), ("integral_tol", C.c_double),
-                ("final_fp64", C.c_int), ("density_tol", C.c_double), ("reserved", C.c_int * 5)]
+                ("final_fp64", C.c_int), ("density_tol", C.c_double), ("graph", C.c_int), ("autotune", C.c_int),
+                ("reserved", C.c_int * 3)]
 
 
 class ScfResult(C.Structure):
     _fields_ = [("energy", C.c_double), ("energy_elec", C.c_double), ("energy_nuc", C.c_double),
                 ("iterations", C.c_int), ("converged", C.c_int), ("fock_seconds", C.c_double),
                 ("last_fock_seconds", C.c_double), ("fock_builds", C.c_int), ("energy_final", C.c_double)]
+
+
+class BatchStats(C.Structure):
+    _fields_ = [("setup_seconds", C.c_double), ("scf_seconds", C.c_double), ("fock_seconds", C.c_double),
+                ("gradient_seconds", C.c_double), ("autotune_seconds", C.c_double), ("iterations", C.c_int),
+                ("work_items", C.c_int), ("quartets", C.c_longlong)]
 
 
 JKFUNC = C.CFUNCTYPE(C.c_int, C.c_void_p, _dp, _dp, _dp)
@@ -81,6 +88,13 @@ def hfgpu():
         lib.hfg_fock.argtypes = [C.c_void_p, _dp, _dp, C.c_int, _dp]
         lib.hfg_reset_incremental.argtypes = [C.c_void_p]
         lib.hfg_set_precision.argtypes = [C.c_void_p, C.c_int]
+        lib.hfg_batch_add_molecule.argtypes = [C.c_void_p, C.c_int, _ip, _ip, _ip, _dp, _dp, C.c_int, _ip, _ip,
+                                               C.c_int, _dp]
+        lib.hfg_batch_commit.argtypes = [C.c_void_p]
+        lib.hfg_batch_layout.argtypes = [C.c_void_p, _ip, _ip, C.POINTER(C.c_longlong), _ip]
+        lib.hfg_autotune.argtypes = [C.c_void_p, C.c_int]
+        lib.hfg_class_stats.argtypes = [C.c_void_p, C.c_int, _ip, C.POINTER(C.c_longlong), _dp, _dp]
+        lib.hfg_launch_shapes.argtypes = [C.c_void_p, C.c_int, _ip, _ip, _ip, _ip, _ip]
         lib.hfg_energy.argtypes = [C.c_void_p, _dp, _dp]
         lib.hfg_energy_backward.argtypes = [C.c_void_p, _dp, _dp]
         lib.hfg_eri_shell_quartet.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, _dp]
@@ -116,6 +130,8 @@ def hfscf():
         lib.hfscf_rhf_run.argtypes = [C.c_void_p, C.POINTER(ScfOptions), C.POINTER(Callbacks),
                                       C.POINTER(ScfResult), _dp, _dp, _dp]
         lib.hfscf_rhf_gradient.argtypes = [C.c_void_p, C.POINTER(ScfOptions), C.POINTER(Callbacks), _dp, _dp, _dp]
+        lib.hfscf_batch_rhf.argtypes = [C.c_int, C.POINTER(C.c_void_p), C.POINTER(ScfOptions), C.POINTER(ScfResult),
+                                        _dp, _dp, _dp, C.POINTER(BatchStats)]
         lib.hfscf_last_error.restype = C.c_char_p
         lib._typed = True
     return lib
@@ -202,6 +218,31 @@ class Molecule:
         return g
 
 
+ANGSTROM_TO_BOHR = 1.8897261254535  # hfscf: src/molecule/hfscf_constants.hpp:5
+_SYMBOLS = {"H": 1, "HE": 2, "LI": 3, "BE": 4, "B": 5, "C": 6, "N": 7, "O": 8, "F": 9, "NE": 10}
+
+
+def read_xyz_frames(path):
+    """Frames of a multi-frame .xyz file (angstrom) as (Z, xyz_bohr) pairs."""
+    frames = []
+    with open(path) as f:
+        lines = f.read().splitlines()
+    k = 0
+    while k < len(lines):
+        if not lines[k].strip():
+            k += 1
+            continue
+        nat = int(lines[k])
+        Z, X = [], []
+        for line in lines[k + 2:k + 2 + nat]:
+            sym, x, y, z = line.split()[:4]
+            Z.append(_SYMBOLS[sym.upper()])
+            X.append([float(x), float(y), float(z)])
+        frames.append((np.array(Z, dtype=np.int32), np.array(X) * ANGSTROM_TO_BOHR))
+        k += 2 + nat
+    return frames
+
+
 def scf_options(**kw):
     o = ScfOptions()
     hfscf().hfscf_default_scf_options(C.byref(o))
@@ -268,22 +309,75 @@ def rhf_gradient(mol, D, F, jk=None, grad2e=None, **opts):
     return g
 
 
-class GpuContext:
-    """The GPU two-electron engine for one molecule (hfgpu C ABI)."""
+def batch_rhf(mols, gradient=False, **opts):
+    """Batched RHF of many molecules (lockstep SCF, one batched GPU Fock build per iteration).
+    Returns (list of per-molecule result dicts with D, F and optionally the gradient, stats)."""
+    lib = hfscf()
+    o = scf_options(**opts)
+    n = len(mols)
+    arr = (C.c_void_p * n)(*[m._h for m in mols])
+    res = (ScfResult * n)()
+    sizes = [m.nbf * m.nbf for m in mols]
+    offs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    D = np.zeros(int(offs[-1]))
+    F = np.zeros(int(offs[-1]))
+    goffs = np.concatenate([[0], np.cumsum([3 * m.natom for m in mols])]).astype(np.int64)
+    G = np.zeros(int(goffs[-1])) if gradient else None
+    st = BatchStats()
+    rc = lib.hfscf_batch_rhf(n, arr, C.byref(o), res, _ptr(D), _ptr(F), _ptr(G) if gradient else None, C.byref(st))
+    if rc not in (0, -2):
+        raise RuntimeError(lib.hfscf_last_error().decode())
+    out = []
+    for k, m in enumerate(mols):
+        r = res[k]
+        d = dict(energy=r.energy, energy_final=r.energy_final, iterations=r.iterations, converged=bool(r.converged),
+                 D=D[offs[k]:offs[k + 1]].reshape(m.nbf, m.nbf), F=F[offs[k]:offs[k + 1]].reshape(m.nbf, m.nbf))
+        if gradient:
+            d["gradient"] = G[goffs[k]:goffs[k + 1]].reshape(m.natom, 3)
+        out.append(d)
+    return out, {k: getattr(st, k) for k, _ in BatchStats._fields_}
 
-    def __init__(self, mol, precision=FP64, integral_tol=1e-14, schwarz=True, device=0, density_tol=0.0):
+
+class GpuContext:
+    """The GPU two-electron engine for one molecule, or for a batch of molecules when given a
+    list (hfgpu C ABI; batched matrices are concatenated per molecule)."""
+
+    def __init__(self, mol, precision=FP64, integral_tol=1e-14, schwarz=True, device=0, density_tol=0.0,
+                 graph=False):
         lib = hfgpu()
         opt = HfgOptions()
         lib.hfg_default_options(C.byref(opt))
         opt.precision, opt.integral_tol, opt.schwarz = precision, integral_tol, int(schwarz)
         opt.density_tol = density_tol
+        opt.graph = int(graph)
         h = C.c_void_p()
         rc = lib.hfg_create(C.byref(h), device, C.byref(opt))
         if rc:
             raise RuntimeError(f"hfg_create failed with status {rc} (no usable GPU?)")
         self._h = h
+        if isinstance(mol, (list, tuple)):
+            self.mols = list(mol)
+            self.mol = None
+            for m in self.mols:
+                sh = m.shells()
+                x = np.ascontiguousarray(m.geometry()[1], dtype=np.float64).reshape(-1)
+                rc = lib.hfg_batch_add_molecule(h, m.nshell, sh["L"].ctypes.data_as(_ip),
+                                                sh["atom"].ctypes.data_as(_ip), sh["nprim"].ctypes.data_as(_ip),
+                                                _ptr(sh["alpha"]), _ptr(sh["coef"]), int(m.pure),
+                                                sh["cart_off"].ctypes.data_as(_ip),
+                                                sh["pure_off"].ctypes.data_as(_ip), m.natom, _ptr(x))
+                if rc < 0:
+                    self._check(rc)
+            self._check(lib.hfg_batch_commit(h))
+            self.nbf = None
+            self.sizes = [m.nbf * m.nbf for m in self.mols]
+            self.offsets = np.concatenate([[0], np.cumsum(self.sizes)]).astype(np.int64)
+            self.natom = sum(m.natom for m in self.mols)
+            return
+        self.mols = [mol]
         self.mol = mol
         self.nbf = mol.nbf
+        self.natom = mol.natom
         sh = mol.shells()
         self._check(lib.hfg_set_basis(h, mol.nshell, sh["L"].ctypes.data_as(_ip), sh["atom"].ctypes.data_as(_ip),
                                       sh["nprim"].ctypes.data_as(_ip), _ptr(sh["alpha"]), _ptr(sh["coef"]),
@@ -300,20 +394,47 @@ class GpuContext:
         x = np.ascontiguousarray(xyz_bohr, dtype=np.float64).reshape(-1)
         self._check(hfgpu().hfg_set_geometry(self._h, len(Z), _ptr(x), Z.ctypes.data_as(_ip)))
 
+    def _like(self, D):
+        return np.zeros((self.nbf, self.nbf)) if self.nbf is not None else np.zeros(int(self.offsets[-1]))
+
     def build_jk(self, D):
-        n = self.nbf
         D = np.ascontiguousarray(D, dtype=np.float64)
-        J, K = np.zeros((n, n)), np.zeros((n, n))
+        J, K = self._like(D), self._like(D)
         self._check(hfgpu().hfg_build_jk(self._h, _ptr(D), _ptr(J), _ptr(K), 0))
         return J, K
 
     def fock(self, H, D, incremental=False):
-        n = self.nbf
-        F = np.zeros((n, n))
+        F = self._like(D)
         H = np.ascontiguousarray(H, dtype=np.float64)
         D = np.ascontiguousarray(D, dtype=np.float64)
         self._check(hfgpu().hfg_fock(self._h, _ptr(H), _ptr(D), int(incremental), _ptr(F)))
         return F
+
+    def split(self, flat):
+        """Per-molecule matrices of a concatenated (batch) array."""
+        return [flat[self.offsets[k]:self.offsets[k + 1]].reshape(m.nbf, m.nbf) for k, m in enumerate(self.mols)]
+
+    def class_stats(self):
+        """Per class of the last direct J/K build: quartets, model FLOPs, kernel ms."""
+        lib = hfgpu()
+        n = lib.hfg_class_stats(self._h, 0, None, None, None, None)
+        cls = np.zeros(n, dtype=np.int32)
+        q = np.zeros(n, dtype=np.int64)
+        fl, ms = np.zeros(n), np.zeros(n)
+        self._check(min(0, lib.hfg_class_stats(self._h, n, cls.ctypes.data_as(_ip),
+                                               q.ctypes.data_as(C.POINTER(C.c_longlong)), _ptr(fl), _ptr(ms))))
+        return [dict(cls=f"{c:04d}", quartets=int(a), model_flops=float(b), kernel_ms=float(d))
+                for c, a, b, d in zip(cls, q, fl, ms)]
+
+    def autotune(self, what=1):
+        self._check(hfgpu().hfg_autotune(self._h, int(what)))
+
+    def launch_shapes(self):
+        lib = hfgpu()
+        n = lib.hfg_launch_shapes(self._h, 0, None, None, None, None, None)
+        a = [np.zeros(n, dtype=np.int32) for _ in range(5)]
+        lib.hfg_launch_shapes(self._h, n, *[x.ctypes.data_as(_ip) for x in a])
+        return [dict(cls=f"{c:04d}", jk=(int(g), int(b)), grad=(int(gg), int(gb))) for c, g, b, gg, gb in zip(*a)]
 
     def set_precision(self, precision):
         self._check(hfgpu().hfg_set_precision(self._h, int(precision)))
@@ -325,7 +446,7 @@ class GpuContext:
         return e.value
 
     def energy_backward(self, D):
-        g = np.zeros((self.mol.natom, 3))
+        g = np.zeros((self.natom, 3))
         D = np.ascontiguousarray(D, dtype=np.float64)
         self._check(hfgpu().hfg_energy_backward(self._h, _ptr(D), _ptr(g)))
         return g

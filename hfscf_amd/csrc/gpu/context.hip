@@ -1,5 +1,6 @@
-// Host side of the hfgpu C ABI: set-up of blocks, primitive pairs, Schwarz screening,
-// work decomposition and gather lists; J/K, Fock, energy and gradient drivers.
+// Host side of the hfgpu C ABI: set-up of blocks, primitive pairs, Schwarz screening, work
+// decomposition and gather lists for one molecule or a batch of molecules; the J/K, Fock,
+// energy and gradient drivers; HIP-graph replay of the J/K build and the launch-shape autotuner.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -34,28 +35,32 @@ int ket_count_host(const double* s, int n, double si, double tol) {
     return lo;
 }
 
+// Blocks: shells of one molecule, one angular momentum and one primitive count, at most
+// kBlockFuncs output functions each. Every shell pair of a block pair then has the same number
+// of primitive pairs, so the lanes of a wavefront run primitive loops of equal length. Blocks
+// of a molecule are contiguous.
 void build_blocks(hfg_ctx* c) {
     std::vector<int> order(c->nshell);
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+        if (c->shell_mol[x] != c->shell_mol[y]) return c->shell_mol[x] < c->shell_mol[y];
         if (c->L[x] != c->L[y]) return c->L[x] < c->L[y];
         return c->nprim[x] > c->nprim[y];
     });
     c->blk_L.clear();
     c->blk_nf.clear();
+    c->blk_mol.clear();
     c->blk_shells.clear();
     c->shell_blk.assign(c->nshell, -1);
     c->shell_slot.assign(c->nshell, -1);
     size_t k = 0;
     while (k < order.size()) {
-        // one angular momentum and one primitive count per block: every shell pair of a block
-        // pair then has the same number of primitive pairs, so the lanes of a wavefront run
-        // primitive loops of equal length
-        const int Lb = c->L[order[k]], np = c->nprim[order[k]];
+        const int Lb = c->L[order[k]], m = c->shell_mol[order[k]], np = c->nprim[order[k]];
         const int nf = nfun(Lb, c->pure);
         const int per = std::max(1, kBlockFuncs / nf);
         std::vector<int> sh;
-        while (k < order.size() && c->L[order[k]] == Lb && c->nprim[order[k]] == np && (int)sh.size() < per)
+        while (k < order.size() && c->L[order[k]] == Lb && c->shell_mol[order[k]] == m && c->nprim[order[k]] == np &&
+               (int)sh.size() < per)
             sh.push_back(order[k++]);
         const int b = (int)c->blk_L.size();
         for (size_t s = 0; s < sh.size(); ++s) {
@@ -64,9 +69,15 @@ void build_blocks(hfg_ctx* c) {
         }
         c->blk_L.push_back(Lb);
         c->blk_nf.push_back((int)sh.size() * nf);
+        c->blk_mol.push_back(m);
         c->blk_shells.push_back(sh);
     }
     c->nblk = (int)c->blk_L.size();
+    for (auto& M : c->mols) M.nblk = 0;
+    for (int b = c->nblk - 1; b >= 0; --b) {
+        c->mols[c->blk_mol[b]].blk0 = b;
+        c->mols[c->blk_mol[b]].nblk++;
+    }
     c->blk_func_off.assign(c->nblk + 1, 0);
     c->blk_func_ao.clear();
     for (int b = 0; b < c->nblk; ++b) {
@@ -79,13 +90,29 @@ void build_blocks(hfg_ctx* c) {
     }
 }
 
-// Primitive-pair data of shell pair (a, b) appended to pp0/pp1/pp2.
+// Block pair index of blocks R >= C of one molecule.
+inline int bp_index(const hfg_ctx* c, int R, int C) {
+    const MolHost& M = c->mols[c->blk_mol[R]];
+    const int r = R - M.blk0, q = C - M.blk0;
+    return M.bp0 + r * (r + 1) / 2 + q;
+}
+
+// A candidate shell pair: device record plus its global shell ids and the offset of its
+// Cartesian Schwarz diagonal.
+struct CandPair {
+    PairDev p;
+    int ga, gb;
+    long long diag_off;
+};
+
+// Primitive-pair data of global shells (a, b) appended to pp0/pp1/pp2.
 PairDev make_pair(hfg_ctx* c, int a, int b, std::vector<double4>& pp0, std::vector<double4>& pp1,
                   std::vector<double4>& pp2) {
     static const double kSqrt2Pi25 = std::sqrt(2.0 * std::pow(M_PI, 2.5));
+    const int s0 = c->mols[c->shell_mol[a]].shell0;
     PairDev p{};
-    p.sa = a;
-    p.sb = b;
+    p.sa = a - s0;
+    p.sb = b - s0;
     p.slot_a = c->shell_slot[a];
     p.slot_b = c->shell_slot[b];
     p.ao_a = c->pure ? c->pure_off[a] : c->cart_off[a];
@@ -98,6 +125,7 @@ PairDev make_pair(hfg_ctx* c, int a, int b, std::vector<double4>& pp0, std::vect
         r2 += p.AB[d] * p.AB[d];
     }
     p.off = (int)pp0.size();
+    p.pstride = 1;
     p.np = c->nprim[a] * c->nprim[b];
     for (int i = 0; i < c->nprim[a]; ++i)
         for (int j = 0; j < c->nprim[b]; ++j) {
@@ -114,117 +142,192 @@ PairDev make_pair(hfg_ctx* c, int a, int b, std::vector<double4>& pp0, std::vect
     return p;
 }
 
+void destroy_graphs(hfg_ctx* c) {
+    for (auto& g : c->graphs)
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    c->graphs.clear();
+}
+
 int setup_geometry(hfg_ctx* c) {
+    destroy_graphs(c);
     build_blocks(c);
-    const int nb = c->nblk;
-    const int nbp = nb * (nb + 1) / 2;
-    // candidate pairs per block pair
-    std::vector<std::vector<PairDev>> cand(nbp);
+    // block pairs (X >= Y) of every molecule
+    c->bp_X.clear();
+    c->bp_Y.clear();
+    for (auto& M : c->mols) {
+        M.bp0 = (int)c->bp_X.size();
+        for (int X = M.blk0; X < M.blk0 + M.nblk; ++X)
+            for (int Y = M.blk0; Y <= X; ++Y) {
+                c->bp_X.push_back(X);
+                c->bp_Y.push_back(Y);
+            }
+        M.nbp = (int)c->bp_X.size() - M.bp0;
+    }
+    const int nbp = (int)c->bp_X.size();
+    // candidate pairs per block pair, with their Cartesian diagonal slots
+    std::vector<std::vector<CandPair>> cand(nbp);
     std::vector<double4> pp0, pp1, pp2;
-    for (int X = 0; X < nb; ++X)
-        for (int Y = 0; Y <= X; ++Y) {
-            auto& v = cand[X * (X + 1) / 2 + Y];
-            for (size_t i = 0; i < c->blk_shells[X].size(); ++i)
-                for (size_t j = 0; j < c->blk_shells[Y].size(); ++j) {
-                    if (X == Y && j > i) continue;
-                    v.push_back(make_pair(c, c->blk_shells[X][i], c->blk_shells[Y][j], pp0, pp1, pp2));
-                }
-        }
+    long long ndiag = 0;
+    for (int g = 0; g < nbp; ++g) {
+        const int X = c->bp_X[g], Y = c->bp_Y[g];
+        auto& v = cand[g];
+        for (size_t i = 0; i < c->blk_shells[X].size(); ++i)
+            for (size_t j = 0; j < c->blk_shells[Y].size(); ++j) {
+                if (X == Y && j > i) continue;
+                const int a = c->blk_shells[X][i], b = c->blk_shells[Y][j];
+                CandPair cp{make_pair(c, a, b, pp0, pp1, pp2), a, b, ndiag};
+                ndiag += (long long)fm_ncart(c->L[a]) * fm_ncart(c->L[b]);
+                v.push_back(cp);
+            }
+    }
     HFG_CHECK(c, c->d_pp0.upload(pp0, c->stream));
     HFG_CHECK(c, c->d_pp1.upload(pp1, c->stream));
     HFG_CHECK(c, c->d_pp2.upload(pp2, c->stream));
-    // Schwarz diagonal on the GPU, grouped by pair class
+    // Cartesian Schwarz diagonals (pq|pq) of every candidate pair on the GPU, grouped by class
+    std::vector<double> diag((size_t)std::max<long long>(ndiag, 1), 0.0);
     {
         std::vector<PairDev> sp;
-        std::vector<int> ca, cb;
+        std::vector<long long> off;
         std::vector<std::pair<int, int>> cls_range;  // (la*3+lb, begin)
-        std::vector<int> cls_l;
         for (int la = 0; la <= 2; ++la)
             for (int lb = 0; lb <= la; ++lb) {
                 const int begin = (int)sp.size();
                 for (auto& v : cand)
-                    for (auto& p : v)
-                        if (c->L[p.sa] == la && c->L[p.sb] == lb) {
-                            sp.push_back(p);
-                            ca.push_back(c->cart_off[p.sa]);
-                            cb.push_back(c->cart_off[p.sb]);
+                    for (auto& cp : v)
+                        if (c->L[cp.ga] == la && c->L[cp.gb] == lb) {
+                            sp.push_back(cp.p);
+                            off.push_back(cp.diag_off);
                         }
                 cls_range.push_back({la * 3 + lb, begin});
             }
         cls_range.push_back({-1, (int)sp.size()});
         DevBuf<PairDev> dsp;
-        DevBuf<int> dca, dcb;
+        DevBuf<long long> doff;
+        DevBuf<double> ddiag;
         HFG_CHECK(c, dsp.upload(sp, c->stream));
-        HFG_CHECK(c, dca.upload(ca, c->stream));
-        HFG_CHECK(c, dcb.upload(cb, c->stream));
-        HFG_CHECK(c, c->d_slab.ensure((size_t)c->ncart * c->ncart));
-        HFG_CHECK(c, hipMemsetAsync(c->d_slab.p, 0, (size_t)c->ncart * c->ncart * sizeof(double), c->stream));
+        HFG_CHECK(c, doff.upload(off, c->stream));
+        HFG_CHECK(c, ddiag.ensure(diag.size()));
         for (size_t r = 0; r + 1 < cls_range.size(); ++r) {
             ScreenArgs a{};
             a.pairs = dsp.p + cls_range[r].second;
-            a.cart_a = dca.p + cls_range[r].second;
-            a.cart_b = dcb.p + cls_range[r].second;
+            a.out_off = doff.p + cls_range[r].second;
             a.count = cls_range[r + 1].second - cls_range[r].second;
             a.pp0 = c->d_pp0.p;
             a.pp1 = c->d_pp1.p;
             a.pp2 = c->d_pp2.p;
             a.boys = c->d_boys.p;
-            a.ncart = c->ncart;
-            a.screen = c->d_slab.p;
+            a.out = ddiag.p;
             HFG_CHECK(c, launch_screen(cls_range[r].first / 3, cls_range[r].first % 3, a, c->stream));
         }
-        c->screen.assign((size_t)c->ncart * c->ncart, 0.0);
-        HFG_CHECK(c, hipMemcpyAsync(c->screen.data(), c->d_slab.p, c->screen.size() * sizeof(double),
-                                    hipMemcpyDeviceToHost, c->stream));
+        HFG_CHECK(c, hipMemcpyAsync(diag.data(), ddiag.p, diag.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                    c->stream));
         HFG_CHECK(c, hipStreamSynchronize(c->stream));
         dsp.release();
-        dca.release();
-        dcb.release();
+        doff.release();
+        ddiag.release();
+    }
+    // the ncart x ncart Schwarz matrix of a single molecule (hfg_schwarz)
+    c->screen.clear();
+    if (c->mols.size() == 1) {
+        const int nc = c->mols[0].ncart;
+        c->screen.assign((size_t)nc * nc, 0.0);
+        for (auto& v : cand)
+            for (auto& cp : v) {
+                const int na = fm_ncart(c->L[cp.ga]), nb = fm_ncart(c->L[cp.gb]);
+                const int ca = c->cart_off[cp.ga], cb = c->cart_off[cp.gb];
+                for (int x = 0; x < na; ++x)
+                    for (int y = 0; y < nb; ++y) {
+                        const double val = diag[cp.diag_off + x * nb + y];
+                        c->screen[(size_t)(ca + x) * nc + cb + y] = val;
+                        c->screen[(size_t)(cb + y) * nc + ca + x] = val;
+                    }
+            }
     }
     // mean Schwarz value per pair (hfscf's shell skip compares the mean of sqrt|S_pq S_rs|,
     // which factorises into a product of per-pair means; hfscf_oseri.cpp:36-46)
     c->tol = c->opt.schwarz ? 100.0 * c->opt.integral_tol : 0.0;
-    std::vector<std::vector<std::pair<double, PairDev>>> lists(nbp);
-    double smax = 0;
-    for (int P = 0; P < nbp; ++P)
-        for (auto& p : cand[P]) {
-            const int ma = (c->L[p.sa] + 1) * (c->L[p.sa] + 2) / 2, mb = (c->L[p.sb] + 1) * (c->L[p.sb] + 2) / 2;
+    std::vector<std::vector<std::pair<double, const CandPair*>>> lists(nbp);
+    std::vector<double> smax(c->mols.size(), 0.0);
+    for (int g = 0; g < nbp; ++g)
+        for (auto& cp : cand[g]) {
+            const int na = fm_ncart(c->L[cp.ga]), nb = fm_ncart(c->L[cp.gb]);
             double s = 0;
-            for (int x = 0; x < ma; ++x)
-                for (int y = 0; y < mb; ++y)
-                    s += std::sqrt(std::fabs(c->screen[(size_t)(c->cart_off[p.sa] + x) * c->ncart + c->cart_off[p.sb] + y]));
-            s /= (double)(ma * mb);
-            smax = std::max(smax, s);
-            lists[P].push_back({s, p});
+            for (int k = 0; k < na * nb; ++k) s += std::sqrt(std::fabs(diag[cp.diag_off + k]));
+            s /= (double)(na * nb);
+            const int m = c->shell_mol[cp.ga];
+            smax[m] = std::max(smax[m], s);
+            lists[g].push_back({s, &cp});
         }
     c->pairs.clear();
     c->psmean.clear();
     c->psq.clear();
     c->bp_off.assign(nbp, 0);
     c->bp_cnt.assign(nbp, 0);
-    for (int P = 0; P < nbp; ++P) {
-        auto& v = lists[P];
+    for (int g = 0; g < nbp; ++g) {
+        auto& v = lists[g];
+        const int m = c->blk_mol[c->bp_X[g]];
         std::stable_sort(v.begin(), v.end(), [](const auto& x, const auto& y) { return x.first > y.first; });
-        c->bp_off[P] = (int)c->pairs.size();
+        c->bp_off[g] = (int)c->pairs.size();
         for (auto& e : v) {
-            if (e.first * smax < c->tol) continue;
-            PairDev pd = e.second;
-            const int ma = (c->L[pd.sa] + 1) * (c->L[pd.sa] + 2) / 2, mb = (c->L[pd.sb] + 1) * (c->L[pd.sb] + 2) / 2;
+            // a pair whose mean times the molecule's largest mean is below the threshold has no
+            // surviving quartet
+            if (e.first * smax[m] < c->tol) continue;
+            PairDev pd = e.second->p;
+            const int na = fm_ncart(c->L[e.second->ga]), nb = fm_ncart(c->L[e.second->gb]);
             pd.sq_off = (int)c->psq.size();
             pd.sq_min = 1e300;
             pd.sq_max = 0.0;
-            for (int x = 0; x < ma; ++x)
-                for (int y = 0; y < mb; ++y) {
-                    const double q = std::sqrt(
-                        std::fabs(c->screen[(size_t)(c->cart_off[pd.sa] + x) * c->ncart + c->cart_off[pd.sb] + y]));
-                    c->psq.push_back(q);
-                    pd.sq_min = std::min(pd.sq_min, q);
-                    pd.sq_max = std::max(pd.sq_max, q);
-                }
+            for (int k = 0; k < na * nb; ++k) {
+                const double q = std::sqrt(std::fabs(diag[e.second->diag_off + k]));
+                c->psq.push_back(q);
+                pd.sq_min = std::min(pd.sq_min, q);
+                pd.sq_max = std::max(pd.sq_max, q);
+            }
             c->pairs.push_back(pd);
             c->psmean.push_back(e.first);
         }
-        c->bp_cnt[P] = (int)c->pairs.size() - c->bp_off[P];
+        c->bp_cnt[g] = (int)c->pairs.size() - c->bp_off[g];
+    }
+    // primitive data of the surviving pairs, primitive-major within each block pair's list
+    {
+        size_t tot = 0;
+        for (const auto& pd : c->pairs) tot += pd.np;
+        std::vector<double4> q0(std::max<size_t>(tot, 1)), q1(q0.size()), q2(q0.size());
+        size_t pos = 0;
+        for (int g = 0; g < nbp; ++g) {
+            const int cnt = c->bp_cnt[g];
+            if (!cnt) continue;
+            PairDev* lst = &c->pairs[c->bp_off[g]];
+            bool uniform = true;
+            for (int j = 1; j < cnt; ++j) uniform = uniform && lst[j].np == lst[0].np;
+            if (uniform) {
+                const int np = lst[0].np;
+                for (int j = 0; j < cnt; ++j) {
+                    for (int k = 0; k < np; ++k) {
+                        q0[pos + (size_t)k * cnt + j] = pp0[lst[j].off + k];
+                        q1[pos + (size_t)k * cnt + j] = pp1[lst[j].off + k];
+                        q2[pos + (size_t)k * cnt + j] = pp2[lst[j].off + k];
+                    }
+                    lst[j].off = (int)(pos + j);
+                    lst[j].pstride = cnt;
+                }
+                pos += (size_t)np * cnt;
+            } else {
+                for (int j = 0; j < cnt; ++j) {
+                    for (int k = 0; k < lst[j].np; ++k) {
+                        q0[pos + k] = pp0[lst[j].off + k];
+                        q1[pos + k] = pp1[lst[j].off + k];
+                        q2[pos + k] = pp2[lst[j].off + k];
+                    }
+                    lst[j].off = (int)pos;
+                    lst[j].pstride = 1;
+                    pos += lst[j].np;
+                }
+            }
+        }
+        HFG_CHECK(c, c->d_pp0.upload(q0, c->stream));
+        HFG_CHECK(c, c->d_pp1.upload(q1, c->stream));
+        HFG_CHECK(c, c->d_pp2.upload(q2, c->stream));
     }
     // work items per class
     std::vector<double> cum_np(c->pairs.size() + nbp, 0.0);  // per block pair prefix of np
@@ -237,11 +340,12 @@ int setup_geometry(hfg_ctx* c) {
             for (int i = 0; i < c->bp_cnt[P]; ++i, ++k) cum_np[k] = cum_np[k - 1] + c->pairs[c->bp_off[P] + i].np;
         }
     }
-    struct Row {
-        int cnt;
-        double cost;
-    };
+    for (auto& cw : c->classes) {
+        if (cw.ev_start) (void)hipEventDestroy(cw.ev_start);
+        if (cw.ev_stop) (void)hipEventDestroy(cw.ev_stop);
+    }
     c->classes.clear();
+    c->class_timed = false;
     auto class_index = [&](int la, int lb, int lc, int ld) {
         for (size_t i = 0; i < c->classes.size(); ++i) {
             auto& w = c->classes[i];
@@ -258,109 +362,101 @@ int setup_geometry(hfg_ctx* c) {
     // pass 1: total cost and cost per class
     double total_cost = 0;
     std::vector<double> class_cost;
-    auto bpX = [&](int P) {
-        int X = 0;
-        while ((X + 1) * (X + 2) / 2 <= P) ++X;
-        return X;
-    };
-    for (int P = 0; P < nbp; ++P)
-        for (int Q = 0; Q <= P; ++Q) {
-            if (!c->bp_cnt[P] || !c->bp_cnt[Q]) continue;
-            const int A = bpX(P), B = P - A * (A + 1) / 2, C = bpX(Q), D = Q - C * (C + 1) / 2;
-            const FlopParts fp = model_flops(c->blk_L[A], c->blk_L[B], c->blk_L[C], c->blk_L[D], c->pure, false);
-            const int ci = class_index(c->blk_L[A], c->blk_L[B], c->blk_L[C], c->blk_L[D]);
-            if ((int)class_cost.size() <= ci) class_cost.resize(ci + 1, 0.0);
-            for (int i = 0; i < c->bp_cnt[P]; ++i) {
-                int cnt = ket_count_host(&c->psmean[c->bp_off[Q]], c->bp_cnt[Q], c->psmean[c->bp_off[P] + i], c->tol);
-                if (P == Q) cnt = std::min(cnt, i + 1);
-                if (!cnt) continue;
-                const double cost =
-                    c->pairs[c->bp_off[P] + i].np * cum_np[cum_off[Q] + cnt] * fp.per_prim + cnt * fp.per_quartet;
-                total_cost += cost;
-                class_cost[ci] += cost;
+    for (const auto& M : c->mols)
+        for (int P = M.bp0; P < M.bp0 + M.nbp; ++P)
+            for (int Q = M.bp0; Q <= P; ++Q) {
+                if (!c->bp_cnt[P] || !c->bp_cnt[Q]) continue;
+                const int A = c->bp_X[P], B = c->bp_Y[P], C = c->bp_X[Q], D = c->bp_Y[Q];
+                const FlopParts fp = model_flops(c->blk_L[A], c->blk_L[B], c->blk_L[C], c->blk_L[D], c->pure, false);
+                const int ci = class_index(c->blk_L[A], c->blk_L[B], c->blk_L[C], c->blk_L[D]);
+                if ((int)class_cost.size() <= ci) class_cost.resize(ci + 1, 0.0);
+                for (int i = 0; i < c->bp_cnt[P]; ++i) {
+                    int cnt =
+                        ket_count_host(&c->psmean[c->bp_off[Q]], c->bp_cnt[Q], c->psmean[c->bp_off[P] + i], c->tol);
+                    if (P == Q) cnt = std::min(cnt, i + 1);
+                    if (!cnt) continue;
+                    const double cost =
+                        c->pairs[c->bp_off[P] + i].np * cum_np[cum_off[Q] + cnt] * fp.per_prim + cnt * fp.per_quartet;
+                    total_cost += cost;
+                    class_cost[ci] += cost;
+                }
             }
-        }
     const double target = std::max(total_cost / (256.0 * 6.0), 1.0);
     c->slab_doubles = 0;
+    c->grad_slab_doubles = 0;
     c->total_quartets = 0;
     c->total_flops = 0;
     c->total_grad_flops = 0;
-    for (int P = 0; P < nbp; ++P)
-        for (int Q = 0; Q <= P; ++Q) {
-            if (!c->bp_cnt[P] || !c->bp_cnt[Q]) continue;
-            const int A = bpX(P), B = P - A * (A + 1) / 2, C = bpX(Q), D = Q - C * (C + 1) / 2;
-            const int ci = class_index(c->blk_L[A], c->blk_L[B], c->blk_L[C], c->blk_L[D]);
-            ClassWork& cw = c->classes[ci];
-            const FlopParts fp = model_flops(cw.la, cw.lb, cw.lc, cw.ld, c->pure, false);
-            const FlopParts gp = model_flops(cw.la, cw.lb, cw.lc, cw.ld, c->pure, true);
-            const int tile = c->blk_nf[A] * c->blk_nf[B] + c->blk_nf[C] * c->blk_nf[D] + c->blk_nf[A] * c->blk_nf[C] +
-                             c->blk_nf[A] * c->blk_nf[D] + c->blk_nf[B] * c->blk_nf[C] + c->blk_nf[B] * c->blk_nf[D];
-            // per-class item target: a class launch should have >= 256 items when it has the work
-            const double tgt = std::max(std::min(target, class_cost[ci] / 256.0), 1.0);
-            WorkItem w{};
-            w.P = P;
-            w.Q = Q;
-            w.c0 = 0;
-            w.c1 = 0x7fffffff;
-            w.A = A;
-            w.B = B;
-            w.C = C;
-            w.D = D;
-            double acc = 0;
-            long long q_acc = 0;
-            int r0 = -1;
-            auto flush = [&](int r1) {
-                if (r0 < 0 || q_acc == 0) {
+    for (int m = 0; m < (int)c->mols.size(); ++m) {
+        const MolHost& M = c->mols[m];
+        for (int P = M.bp0; P < M.bp0 + M.nbp; ++P)
+            for (int Q = M.bp0; Q <= P; ++Q) {
+                if (!c->bp_cnt[P] || !c->bp_cnt[Q]) continue;
+                const int A = c->bp_X[P], B = c->bp_Y[P], C = c->bp_X[Q], D = c->bp_Y[Q];
+                const int ci = class_index(c->blk_L[A], c->blk_L[B], c->blk_L[C], c->blk_L[D]);
+                ClassWork& cw = c->classes[ci];
+                const FlopParts fp = model_flops(cw.la, cw.lb, cw.lc, cw.ld, c->pure, false);
+                const FlopParts gp = model_flops(cw.la, cw.lb, cw.lc, cw.ld, c->pure, true);
+                const int tile = c->blk_nf[A] * c->blk_nf[B] + c->blk_nf[C] * c->blk_nf[D] +
+                                 c->blk_nf[A] * c->blk_nf[C] + c->blk_nf[A] * c->blk_nf[D] +
+                                 c->blk_nf[B] * c->blk_nf[C] + c->blk_nf[B] * c->blk_nf[D];
+                // per-class item target: a class launch should have >= 256 items when it has the work
+                const double tgt = std::max(std::min(target, class_cost[ci] / 256.0), 1.0);
+                WorkItem w{};
+                w.P = P;
+                w.Q = Q;
+                w.A = A;
+                w.B = B;
+                w.C = C;
+                w.D = D;
+                w.mol = m;
+                auto push = [&](int r0, int r1, int c0, int c1) {
+                    WorkItem v = w;
+                    v.r0 = r0;
+                    v.r1 = r1;
+                    v.c0 = c0;
+                    v.c1 = c1;
+                    v.slab = c->slab_doubles;
+                    c->slab_doubles += tile;
+                    v.gslab = c->grad_slab_doubles;
+                    c->grad_slab_doubles += 3 * M.natom;
+                    cw.items.push_back(v);
+                };
+                double acc = 0;
+                long long q_acc = 0;
+                int r0 = -1;
+                auto flush = [&](int r1) {
+                    if (r0 >= 0 && q_acc > 0) push(r0, r1, 0, 0x7fffffff);
                     r0 = -1;
                     acc = 0;
                     q_acc = 0;
-                    return;
-                }
-                w.r0 = r0;
-                w.r1 = r1;
-                w.c0 = 0;
-                w.c1 = 0x7fffffff;
-                w.slab = c->slab_doubles;
-                c->slab_doubles += tile;
-                cw.items.push_back(w);
-                r0 = -1;
-                acc = 0;
-                q_acc = 0;
-            };
-            for (int i = 0; i < c->bp_cnt[P]; ++i) {
-                int cnt = ket_count_host(&c->psmean[c->bp_off[Q]], c->bp_cnt[Q], c->psmean[c->bp_off[P] + i], c->tol);
-                if (P == Q) cnt = std::min(cnt, i + 1);
-                const double np_i = c->pairs[c->bp_off[P] + i].np;
-                const double kq = np_i * cum_np[cum_off[Q] + cnt];
-                cw.quartets += cnt;
-                cw.flops += kq * fp.per_prim + cnt * fp.per_quartet;
-                cw.grad_flops += kq * gp.per_prim + cnt * gp.per_quartet;
-                const double row_cost = kq * fp.per_prim + cnt * fp.per_quartet;
-                if (row_cost > tgt && cnt > 64) {
-                    // an expensive row becomes several items over ket column chunks
-                    flush(i);
-                    const int chunks = std::min((int)std::ceil(row_cost / tgt), (cnt + 31) / 32);
-                    const int width = (cnt + chunks - 1) / chunks;
-                    for (int j0 = 0; j0 < cnt; j0 += width) {
-                        WorkItem v = w;
-                        v.r0 = i;
-                        v.r1 = i + 1;
-                        v.c0 = j0;
-                        v.c1 = std::min(cnt, j0 + width);
-                        v.slab = c->slab_doubles;
-                        c->slab_doubles += tile;
-                        cw.items.push_back(v);
+                };
+                for (int i = 0; i < c->bp_cnt[P]; ++i) {
+                    int cnt =
+                        ket_count_host(&c->psmean[c->bp_off[Q]], c->bp_cnt[Q], c->psmean[c->bp_off[P] + i], c->tol);
+                    if (P == Q) cnt = std::min(cnt, i + 1);
+                    const double np_i = c->pairs[c->bp_off[P] + i].np;
+                    const double kq = np_i * cum_np[cum_off[Q] + cnt];
+                    cw.quartets += cnt;
+                    cw.flops += kq * fp.per_prim + cnt * fp.per_quartet;
+                    cw.grad_flops += kq * gp.per_prim + cnt * gp.per_quartet;
+                    const double row_cost = kq * fp.per_prim + cnt * fp.per_quartet;
+                    if (row_cost > tgt && cnt > 64) {
+                        // an expensive row becomes several items over ket column chunks
+                        flush(i);
+                        const int chunks = std::min((int)std::ceil(row_cost / tgt), (cnt + 31) / 32);
+                        const int width = (cnt + chunks - 1) / chunks;
+                        for (int j0 = 0; j0 < cnt; j0 += width) push(i, i + 1, j0, std::min(cnt, j0 + width));
+                        continue;
                     }
-                    continue;
+                    if (r0 < 0) r0 = i;
+                    acc += row_cost;
+                    q_acc += cnt;
+                    if (acc >= tgt || i + 1 - r0 >= kMaxRows) flush(i + 1);
                 }
-                if (r0 < 0) r0 = i;
-                acc += row_cost;
-                q_acc += cnt;
-                if (acc >= tgt || i + 1 - r0 >= kMaxRows) flush(i + 1);
+                flush(c->bp_cnt[P]);
             }
-            flush(c->bp_cnt[P]);
-        }
-    // heaviest items first inside each class launch
+    }
     for (auto& cw : c->classes) {
         c->total_quartets += cw.quartets;
         c->total_flops += cw.flops;
@@ -386,14 +482,15 @@ int setup_geometry(hfg_ctx* c) {
             load[best] += c->classes[k].flops;
         }
     }
-    // gather lists
-    const int ntile = nbp;
-    std::vector<std::vector<GatherEntry>> lj(ntile), lk(ntile);
+    // gather lists (one output tile per block pair) and per-molecule item lists
+    std::vector<std::vector<GatherEntry>> lj(nbp), lk(nbp);
     std::vector<WorkItem> all;
+    std::vector<std::vector<int>> mitems(c->mols.size());
     c->total_items = 0;
     for (auto& cw : c->classes) {
         cw.dev_off = all.size();
         for (auto& w : cw.items) {
+            mitems[w.mol].push_back((int)all.size());
             all.push_back(w);
             const int f[4] = {c->blk_nf[w.A], c->blk_nf[w.B], c->blk_nf[w.C], c->blk_nf[w.D]};
             const int blk[4] = {w.A, w.B, w.C, w.D};
@@ -402,14 +499,14 @@ int setup_geometry(hfg_ctx* c) {
             long long off = w.slab;
             for (int r = 0; r < 6; ++r) {
                 const int R = blk[rr[r]], C = blk[cc[r]];
-                auto& L = (r < 2) ? lj : lk;
+                auto& Lst = (r < 2) ? lj : lk;
                 if (R > C) {
-                    L[R * (R + 1) / 2 + C].push_back({off, f[cc[r]], 0});
+                    Lst[bp_index(c, R, C)].push_back({off, f[cc[r]], 0});
                 } else if (R < C) {
-                    L[C * (C + 1) / 2 + R].push_back({off, f[cc[r]], 1});
+                    Lst[bp_index(c, C, R)].push_back({off, f[cc[r]], 1});
                 } else {
-                    L[R * (R + 1) / 2 + R].push_back({off, f[cc[r]], 0});
-                    L[R * (R + 1) / 2 + R].push_back({off, f[cc[r]], 1});
+                    Lst[bp_index(c, R, R)].push_back({off, f[cc[r]], 0});
+                    Lst[bp_index(c, R, R)].push_back({off, f[cc[r]], 1});
                 }
                 off += (long long)f[rr[r]] * f[cc[r]];
             }
@@ -421,17 +518,26 @@ int setup_geometry(hfg_ctx* c) {
     c->gK_off.assign(1, 0);
     c->gJ.clear();
     c->gK.clear();
-    for (int X = 0; X < nb; ++X)
-        for (int Y = 0; Y <= X; ++Y) {
-            const int t = X * (X + 1) / 2 + Y;
-            c->tiles.push_back(make_int2(X, Y));
-            c->gJ.insert(c->gJ.end(), lj[t].begin(), lj[t].end());
-            c->gK.insert(c->gK.end(), lk[t].begin(), lk[t].end());
-            c->gJ_off.push_back((int)c->gJ.size());
-            c->gK_off.push_back((int)c->gK.size());
+    for (int g = 0; g < nbp; ++g) {
+        c->tiles.push_back(make_int4(c->bp_X[g], c->bp_Y[g], c->blk_mol[c->bp_X[g]], 0));
+        c->gJ.insert(c->gJ.end(), lj[g].begin(), lj[g].end());
+        c->gK.insert(c->gK.end(), lk[g].begin(), lk[g].end());
+        c->gJ_off.push_back((int)c->gJ.size());
+        c->gK_off.push_back((int)c->gK.size());
+    }
+    std::vector<int> mol_items, mol_items_off(1, 0), out_mol, out_k;
+    for (size_t m = 0; m < c->mols.size(); ++m) {
+        mol_items.insert(mol_items.end(), mitems[m].begin(), mitems[m].end());
+        mol_items_off.push_back((int)mol_items.size());
+        for (int k = 0; k < 3 * c->mols[m].natom; ++k) {
+            out_mol.push_back((int)m);
+            out_k.push_back(k);
         }
+    }
+    std::vector<MolDev> md;
+    for (auto& M : c->mols)
+        md.push_back(MolDev{M.moff, M.nbf, M.natom, M.shell0, M.nshell, M.dm_off, 0});
     // upload
-    std::vector<int> shell_atom(c->atom);
     HFG_CHECK(c, c->d_pairs.upload(c->pairs, c->stream));
     HFG_CHECK(c, c->d_psmean.upload(c->psmean, c->stream));
     HFG_CHECK(c, c->d_psq.upload(c->psq, c->stream));
@@ -446,42 +552,61 @@ int setup_geometry(hfg_ctx* c) {
     HFG_CHECK(c, c->d_gK_off.upload(c->gK_off, c->stream));
     HFG_CHECK(c, c->d_gJ.upload(c->gJ, c->stream));
     HFG_CHECK(c, c->d_gK.upload(c->gK, c->stream));
-    HFG_CHECK(c, c->d_shell_atom.upload(shell_atom, c->stream));
+    HFG_CHECK(c, c->d_shell_atom.upload(c->atom_local, c->stream));
+    HFG_CHECK(c, c->d_mols.upload(md, c->stream));
+    HFG_CHECK(c, c->d_mol_items.upload(mol_items, c->stream));
+    HFG_CHECK(c, c->d_mol_items_off.upload(mol_items_off, c->stream));
+    HFG_CHECK(c, c->d_out_mol.upload(out_mol, c->stream));
+    HFG_CHECK(c, c->d_out_k.upload(out_k, c->stream));
+    HFG_CHECK(c, c->d_queue.ensure(2 * std::max<size_t>(c->classes.size(), 1)));
     HFG_CHECK(c, c->d_slab.ensure((size_t)std::max<long long>(c->slab_doubles, 1)));
-    const size_t nn = (size_t)c->nbf * c->nbf;
+    const size_t nn = (size_t)c->nn_total;
     HFG_CHECK(c, c->d_D.ensure(nn));
     HFG_CHECK(c, c->d_J.ensure(nn));
     HFG_CHECK(c, c->d_K.ensure(nn));
     HFG_CHECK(c, c->d_F.ensure(nn));
     HFG_CHECK(c, c->d_H.ensure(nn));
     HFG_CHECK(c, c->d_G.ensure(nn));
+    size_t ndm = 0;
+    for (auto& M : c->mols) ndm += (size_t)M.nshell * M.nshell;
+    HFG_CHECK(c, c->d_dmax.ensure(ndm));  // allocated once here, so graph arguments stay valid
     HFG_CHECK(c, hipStreamSynchronize(c->stream));
     c->have_prev = false;
     return HFG_OK;
 }
 
-// Largest |D| of every shell block (nshell x nshell), uploaded for the density screen.
+// Largest |D| of every shell block of every molecule, uploaded for the density screen.
 int upload_dmax(hfg_ctx* c, const double* D) {
-    const int ns = c->nshell, n = c->nbf;
-    c->dmax.assign((size_t)ns * ns, 0.0f);
-    for (int a = 0; a < ns; ++a) {
-        const int oa = c->pure ? c->pure_off[a] : c->cart_off[a], na = nfun(c->L[a], c->pure);
-        for (int b = 0; b < ns; ++b) {
-            const int ob = c->pure ? c->pure_off[b] : c->cart_off[b], nb = nfun(c->L[b], c->pure);
-            double m = 0.0;
-            for (int p = 0; p < na; ++p)
-                for (int q = 0; q < nb; ++q) m = std::max(m, std::fabs(D[(size_t)(oa + p) * n + ob + q]));
-            // rounded up so the single-precision table never understates the bound
-            c->dmax[(size_t)a * ns + b] = std::nextafter((float)m, INFINITY);
+    size_t ndm = 0;
+    for (auto& M : c->mols) ndm += (size_t)M.nshell * M.nshell;
+    c->dmax.assign(ndm, 0.0f);
+    for (const auto& M : c->mols) {
+        const int ns = M.nshell, n = M.nbf;
+        const double* Dm = D + M.moff;
+        float* out = c->dmax.data() + M.dm_off;
+        for (int a = 0; a < ns; ++a) {
+            const int ga = M.shell0 + a;
+            const int oa = c->pure ? c->pure_off[ga] : c->cart_off[ga], na = nfun(c->L[ga], c->pure);
+            for (int b = 0; b < ns; ++b) {
+                const int gb = M.shell0 + b;
+                const int ob = c->pure ? c->pure_off[gb] : c->cart_off[gb], nb = nfun(c->L[gb], c->pure);
+                double mx = 0.0;
+                for (int p = 0; p < na; ++p)
+                    for (int q = 0; q < nb; ++q) mx = std::max(mx, std::fabs(Dm[(size_t)(oa + p) * n + ob + q]));
+                // rounded up so the single-precision table never understates the bound
+                out[(size_t)a * ns + b] = mx > 0.0 ? std::nextafter((float)mx, INFINITY) : 0.0f;
+            }
         }
     }
-    HFG_CHECK(c, c->d_dmax.upload(c->dmax, c->stream));
+    if (c->d_dmax.n < ndm) return fail(c, HFG_ERR_ARG, "density table larger than at set-up");
+    HFG_CHECK(c, hipMemcpyAsync(c->d_dmax.p, c->dmax.data(), ndm * sizeof(float), hipMemcpyHostToDevice, c->stream));
     return HFG_OK;
 }
 
-// J and K of the density already in d_D, into d_J / d_K. dscreen: apply the density-weighted
-// screen with the table from upload_dmax.
-int run_jk(hfg_ctx* c, bool dscreen = false) {
+// The launches of one J/K build on `main` (class kernels fork to the side streams and join
+// back): counter reset, class kernels, gather into d_J / d_K. When ev_mid is given it is
+// recorded on main after the join (end of the class kernels).
+int enqueue_jk(hfg_ctx* c, hipStream_t main, bool dscreen, hipEvent_t ev_mid) {
     JkArgs a{};
     a.pairs = c->d_pairs.p;
     a.psmean = c->d_psmean.p;
@@ -493,23 +618,34 @@ int run_jk(hfg_ctx* c, bool dscreen = false) {
     a.pp2 = c->d_pp2.p;
     a.boys = c->d_boys.p;
     a.D = c->d_D.p;
-    a.n = c->nbf;
+    a.mols = c->d_mols.p;
     a.tol = c->tol;
     a.psq = c->d_psq.p;
     a.elem_tol = c->tol;
     a.prim_cut = c->opt.integral_tol / 1e4;
     a.slab = c->d_slab.p;
     a.dmax = c->d_dmax.p;
-    a.nshell = c->nshell;
-    a.dtol = (dscreen && c->d_dmax.p) ? c->opt.density_tol : 0.0;
+    a.dtol = dscreen ? c->opt.density_tol : 0.0;
     const int mixed = c->opt.precision == HFG_MIXED ? 1 : 0;
-    HFG_CHECK(c, hipEventRecord(c->ev0, c->stream));
-    HFG_CHECK(c, for_each_class_concurrent(c, [&](const ClassWork& cw, hipStream_t st) {
+    HFG_CHECK(c, launch_zero_ints(c->d_queue.p, (int)c->classes.size(), main));
+    size_t k = 0;
+    const bool timed = ev_mid != nullptr;  // direct launches: bracket every class launch with events
+    HFG_CHECK(c, for_each_class_concurrent(c, main, [&](ClassWork& cw, hipStream_t st) {
         JkArgs b = a;
         b.items = c->d_items.p + cw.dev_off;
-        return launch_jk(cw.la, cw.lb, cw.lc, cw.ld, c->pure, mixed, b, (int)cw.items.size(), st);
+        b.queue = c->d_queue.p + (k++);
+        if (timed) {
+            hipError_t e = hipSuccess;
+            if (!cw.ev_start && (e = hipEventCreate(&cw.ev_start)) != hipSuccess) return e;
+            if (!cw.ev_stop && (e = hipEventCreate(&cw.ev_stop)) != hipSuccess) return e;
+            if ((e = hipEventRecord(cw.ev_start, st)) != hipSuccess) return e;
+        }
+        hipError_t e = launch_jk(cw.la, cw.lb, cw.lc, cw.ld, c->pure, mixed, b, (int)cw.items.size(), cw.jk_shape, st);
+        if (e == hipSuccess && timed) e = hipEventRecord(cw.ev_stop, st);
+        return e;
     }));
-    HFG_CHECK(c, hipEventRecord(c->ev1, c->stream));
+    c->class_timed = timed;
+    if (ev_mid) HFG_CHECK(c, hipEventRecord(ev_mid, main));
     GatherArgs g{};
     g.tiles = c->d_tiles.p;
     g.gJ_off = c->d_gJ_off.p;
@@ -520,10 +656,47 @@ int run_jk(hfg_ctx* c, bool dscreen = false) {
     g.blk_nf = c->d_blk_nf.p;
     g.blk_func_off = c->d_blk_func_off.p;
     g.blk_func_ao = c->d_blk_func_ao.p;
-    g.n = c->nbf;
+    g.mols = c->d_mols.p;
     g.J = c->d_J.p;
     g.K = c->d_K.p;
-    HFG_CHECK(c, launch_gather(g, (int)c->tiles.size(), c->stream));
+    HFG_CHECK(c, launch_gather(g, (int)c->tiles.size(), main));
+    return HFG_OK;
+}
+
+// J and K of the densities already in d_D, into d_J / d_K, directly or by replaying the
+// captured graph of this configuration. ev0 / ev1 bracket the build (the class kernels only,
+// when launched directly).
+int run_jk(hfg_ctx* c, bool dscreen = false) {
+    if (!c->opt.graph) {
+        HFG_CHECK(c, hipEventRecord(c->ev0, c->stream));
+        return enqueue_jk(c, c->stream, dscreen, c->ev1);
+    }
+    const int mixed = c->opt.precision == HFG_MIXED ? 1 : 0;
+    JkGraph* gr = nullptr;
+    for (auto& g : c->graphs)
+        if (g.valid && g.mixed == mixed && g.dscreen == dscreen) gr = &g;
+    if (!gr) {
+        hipGraph_t graph = nullptr;
+        HFG_CHECK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+        const int r = enqueue_jk(c, c->stream, dscreen, nullptr);
+        const hipError_t e = hipStreamEndCapture(c->stream, &graph);
+        if (r != HFG_OK || e != hipSuccess) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return r != HFG_OK ? r : fail(c, HFG_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+        }
+        JkGraph ng;
+        const hipError_t ei = hipGraphInstantiate(&ng.exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ei != hipSuccess) return fail(c, HFG_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+        ng.valid = true;
+        ng.mixed = mixed;
+        ng.dscreen = dscreen;
+        c->graphs.push_back(ng);
+        gr = &c->graphs.back();
+    }
+    HFG_CHECK(c, hipEventRecord(c->ev0, c->stream));
+    HFG_CHECK(c, hipGraphLaunch(gr->exec, c->stream));
+    HFG_CHECK(c, hipEventRecord(c->ev1, c->stream));
     return HFG_OK;
 }
 
@@ -546,6 +719,83 @@ int check_ready(hfg_ctx* c) {
     return HFG_OK;
 }
 
+// Append one molecule's basis (offsets within the molecule) to the concatenated tables.
+int add_basis(hfg_ctx* c, int nshell, const int* L, const int* atom_of_shell, const int* nprim, const double* alpha,
+              const double* coef, int pure, const int* cart_offset, const int* pure_offset, int atom0) {
+    if (nshell <= 0 || !L || !atom_of_shell || !nprim || !alpha || !coef || !cart_offset || !pure_offset)
+        return fail(c, HFG_ERR_ARG, "basis: bad arguments");
+    if (!c->mols.empty() && c->pure != (pure != 0))
+        return fail(c, HFG_ERR_ARG, "basis: every molecule of a batch must use the same pure/Cartesian choice");
+    MolHost M;
+    M.shell0 = c->nshell;
+    M.nshell = nshell;
+    int tot = (int)c->alpha.size();
+    int ncart = 0, nsph = 0;
+    for (int s = 0; s < nshell; ++s) {
+        if (L[s] < 0 || L[s] > 2) return fail(c, HFG_ERR_UNSUPPORTED, "basis: only s, p and d shells are supported");
+        if (nprim[s] <= 0) return fail(c, HFG_ERR_ARG, "basis: shell without primitives");
+        ncart = std::max(ncart, cart_offset[s] + (L[s] + 1) * (L[s] + 2) / 2);
+        nsph = std::max(nsph, pure_offset[s] + 2 * L[s] + 1);
+    }
+    int nprim_tot = 0;
+    for (int s = 0; s < nshell; ++s) nprim_tot += nprim[s];
+    for (int s = 0; s < nshell; ++s) {
+        c->L.push_back(L[s]);
+        c->atom_local.push_back(atom_of_shell[s]);
+        c->atom.push_back(atom0 + atom_of_shell[s]);
+        c->nprim.push_back(nprim[s]);
+        c->cart_off.push_back(cart_offset[s]);
+        c->pure_off.push_back(pure_offset[s]);
+        c->shell_mol.push_back((int)c->mols.size());
+        c->prim_off.push_back(tot);
+        tot += nprim[s];
+    }
+    c->alpha.insert(c->alpha.end(), alpha, alpha + nprim_tot);
+    c->coef.insert(c->coef.end(), coef, coef + nprim_tot);
+    c->pure = pure != 0;
+    M.ncart = ncart;
+    M.nbf = c->pure ? nsph : ncart;
+    c->nshell += nshell;
+    c->mols.push_back(M);
+    return HFG_OK;
+}
+
+void reset_molecules(hfg_ctx* c) {
+    c->mols.clear();
+    c->nshell = 0;
+    c->L.clear();
+    c->atom.clear();
+    c->atom_local.clear();
+    c->nprim.clear();
+    c->prim_off.clear();
+    c->cart_off.clear();
+    c->pure_off.clear();
+    c->shell_mol.clear();
+    c->alpha.clear();
+    c->coef.clear();
+    c->xyz.clear();
+    c->natom = 0;
+    c->have_basis = c->have_geom = false;
+}
+
+// Matrix and gradient offsets of every molecule.
+void layout_molecules(hfg_ctx* c) {
+    long long moff = 0;
+    int dm = 0, gout = 0;
+    for (auto& M : c->mols) {
+        M.moff = moff;
+        moff += (long long)M.nbf * M.nbf;
+        M.dm_off = dm;
+        dm += M.nshell * M.nshell;
+        M.gout0 = gout;
+        gout += 3 * M.natom;
+    }
+    c->nn_total = moff;
+    c->nout_total = gout;
+    c->nbf = c->mols.empty() ? 0 : c->mols[0].nbf;
+    c->ncart = c->mols.empty() ? 0 : c->mols[0].ncart;
+}
+
 }  // namespace
 
 extern "C" {
@@ -556,6 +806,8 @@ void hfg_default_options(hfg_options* opt) {
     opt->precision = HFG_FP64;
     opt->integral_tol = 1e-14;
     opt->schwarz = 1;
+    opt->density_tol = 0.0;
+    opt->graph = 0;
 }
 
 int hfg_create(hfg_ctx** out, int device, const hfg_options* opt) {
@@ -569,10 +821,14 @@ int hfg_create(hfg_ctx** out, int device, const hfg_options* opt) {
         c->opt = *opt;
     else
         hfg_default_options(&c->opt);
+    if (c->opt.precision != HFG_FP64 && c->opt.precision != HFG_MIXED) {
+        delete c;
+        return HFG_ERR_ARG;
+    }
     if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess ||
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
         hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess) {
-        delete c;
+        hfg_destroy(c);
         return HFG_ERR_HIP;
     }
     for (int k = 0; k < kSideStreams; ++k)
@@ -583,7 +839,7 @@ int hfg_create(hfg_ctx** out, int device, const hfg_options* opt) {
         }
     std::vector<double> tab = boys_table();
     if (c->d_boys.upload(tab, c->stream) != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess) {
-        delete c;
+        hfg_destroy(c);
         return HFG_ERR_HIP;
     }
     *out = c;
@@ -592,46 +848,80 @@ int hfg_create(hfg_ctx** out, int device, const hfg_options* opt) {
 
 int hfg_set_basis(hfg_ctx* c, int nshell, const int* L, const int* atom_of_shell, const int* nprim, const double* alpha,
                   const double* coef, int pure, const int* cart_offset, const int* pure_offset) {
-    if (!c || nshell <= 0 || !L || !atom_of_shell || !nprim || !alpha || !coef || !cart_offset || !pure_offset)
-        return fail(c, HFG_ERR_ARG, "hfg_set_basis: bad arguments");
-    c->nshell = nshell;
-    c->L.assign(L, L + nshell);
-    c->atom.assign(atom_of_shell, atom_of_shell + nshell);
-    c->nprim.assign(nprim, nprim + nshell);
-    c->cart_off.assign(cart_offset, cart_offset + nshell);
-    c->pure_off.assign(pure_offset, pure_offset + nshell);
-    c->pure = pure != 0;
-    c->prim_off.assign(nshell, 0);
-    int tot = 0;
-    int ncart = 0, nsph = 0;
-    for (int s = 0; s < nshell; ++s) {
-        if (L[s] < 0 || L[s] > 2) return fail(c, HFG_ERR_UNSUPPORTED, "hfg_set_basis: only s, p and d shells are supported");
-        if (nprim[s] <= 0) return fail(c, HFG_ERR_ARG, "hfg_set_basis: shell without primitives");
-        c->prim_off[s] = tot;
-        tot += nprim[s];
-        ncart = std::max(ncart, cart_offset[s] + (L[s] + 1) * (L[s] + 2) / 2);
-        nsph = std::max(nsph, pure_offset[s] + 2 * L[s] + 1);
+    if (!c) return HFG_ERR_ARG;
+    if (c->batch) return fail(c, HFG_ERR_ARG, "hfg_set_basis: the context holds a batch of molecules");
+    reset_molecules(c);
+    const int r = add_basis(c, nshell, L, atom_of_shell, nprim, alpha, coef, pure, cart_offset, pure_offset, 0);
+    if (r != HFG_OK) {
+        reset_molecules(c);
+        return r;
     }
-    c->alpha.assign(alpha, alpha + tot);
-    c->coef.assign(coef, coef + tot);
-    c->ncart = ncart;
-    c->nbf = c->pure ? nsph : ncart;
+    layout_molecules(c);
     c->have_basis = true;
-    c->have_geom = false;
     return HFG_OK;
 }
 
 int hfg_set_geometry(hfg_ctx* c, int natom, const double* xyz_bohr, const int* Z) {
     (void)Z;
     if (!c || !c->have_basis || natom <= 0 || !xyz_bohr) return fail(c, HFG_ERR_ARG, "hfg_set_geometry: bad arguments");
+    if (c->batch) return fail(c, HFG_ERR_ARG, "hfg_set_geometry: the context holds a batch of molecules");
     for (int s = 0; s < c->nshell; ++s)
         if (c->atom[s] < 0 || c->atom[s] >= natom) return fail(c, HFG_ERR_ARG, "hfg_set_geometry: shell atom out of range");
     HFG_CHECK(c, hipSetDevice(c->device));
     c->natom = natom;
+    c->mols[0].atom0 = 0;
+    c->mols[0].natom = natom;
     c->xyz.assign(xyz_bohr, xyz_bohr + 3 * natom);
+    layout_molecules(c);
+    c->have_geom = false;
     int r = setup_geometry(c);
     if (r == HFG_OK) c->have_geom = true;
     return r;
+}
+
+int hfg_batch_add_molecule(hfg_ctx* c, int nshell, const int* L, const int* atom_of_shell, const int* nprim,
+                           const double* alpha, const double* coef, int pure, const int* cart_offset,
+                           const int* pure_offset, int natom, const double* xyz_bohr) {
+    if (!c) return HFG_ERR_ARG;
+    if (!c->batch && !c->mols.empty())
+        return fail(c, HFG_ERR_ARG, "hfg_batch_add_molecule: the context already holds a single molecule");
+    if (c->have_geom) return fail(c, HFG_ERR_ARG, "hfg_batch_add_molecule: the batch is already committed");
+    if (natom <= 0 || natom > kMaxAtomsPerMolecule || !xyz_bohr)
+        return fail(c, HFG_ERR_ARG, "hfg_batch_add_molecule: bad atom count or coordinates");
+    for (int s = 0; s < nshell; ++s)
+        if (atom_of_shell && (atom_of_shell[s] < 0 || atom_of_shell[s] >= natom))
+            return fail(c, HFG_ERR_ARG, "hfg_batch_add_molecule: shell atom out of range");
+    c->batch = true;
+    const int r = add_basis(c, nshell, L, atom_of_shell, nprim, alpha, coef, pure, cart_offset, pure_offset, c->natom);
+    if (r != HFG_OK) return r;
+    MolHost& M = c->mols.back();
+    M.atom0 = c->natom;
+    M.natom = natom;
+    c->natom += natom;
+    c->xyz.insert(c->xyz.end(), xyz_bohr, xyz_bohr + 3 * natom);
+    c->have_basis = true;
+    return (int)c->mols.size() - 1;
+}
+
+int hfg_batch_commit(hfg_ctx* c) {
+    if (!c || !c->batch || c->mols.empty()) return fail(c, HFG_ERR_ARG, "hfg_batch_commit: no molecules added");
+    if (c->have_geom) return fail(c, HFG_ERR_ARG, "hfg_batch_commit: already committed");
+    HFG_CHECK(c, hipSetDevice(c->device));
+    layout_molecules(c);
+    int r = setup_geometry(c);
+    if (r == HFG_OK) c->have_geom = true;
+    return r;
+}
+
+int hfg_batch_layout(const hfg_ctx* c, int* nmol, int* nbf, long long* moff, int* natom) {
+    if (!c) return HFG_ERR_ARG;
+    if (nmol) *nmol = (int)c->mols.size();
+    for (size_t m = 0; m < c->mols.size(); ++m) {
+        if (nbf) nbf[m] = c->mols[m].nbf;
+        if (moff) moff[m] = c->mols[m].moff;
+        if (natom) natom[m] = c->mols[m].natom;
+    }
+    return HFG_OK;
 }
 
 int hfg_build_jk(hfg_ctx* c, const double* D, double* J, double* K, int flags) {
@@ -640,7 +930,7 @@ int hfg_build_jk(hfg_ctx* c, const double* D, double* J, double* K, int flags) {
     if (r) return r;
     if (!D) return fail(c, HFG_ERR_ARG, "hfg_build_jk: D is NULL");
     auto t0 = std::chrono::steady_clock::now();
-    const size_t nn = (size_t)c->nbf * c->nbf;
+    const size_t nn = (size_t)c->nn_total;
     HFG_CHECK(c, hipMemcpyAsync(c->d_D.p, D, nn * sizeof(double), hipMemcpyHostToDevice, c->stream));
     if ((r = run_jk(c))) return r;
     if (J) HFG_CHECK(c, hipMemcpyAsync(J, c->d_J.p, nn * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -653,7 +943,7 @@ int hfg_fock(hfg_ctx* c, const double* H, const double* D, int incremental, doub
     if (r) return r;
     if (!H || !D || !F) return fail(c, HFG_ERR_ARG, "hfg_fock: NULL argument");
     auto t0 = std::chrono::steady_clock::now();
-    const size_t nn = (size_t)c->nbf * c->nbf;
+    const size_t nn = (size_t)c->nn_total;
     const bool inc = incremental && c->have_prev;
     std::vector<double> Dd;
     const double* Duse = D;
@@ -687,11 +977,80 @@ int hfg_set_precision(hfg_ctx* c, int precision) {
     return HFG_OK;
 }
 
+int hfg_autotune(hfg_ctx* c, int what) {
+    int r = check_ready(c);
+    if (r) return r;
+    if (what < 1 || what > 3) return fail(c, HFG_ERR_ARG, "hfg_autotune: what must be 1, 2 or 3");
+    // candidate shapes: threads per workgroup x persistent workgroups (0 = one per work item)
+    const int blocks[2] = {256, 128};
+    const int grids[3] = {0, 512, 2048};
+    destroy_graphs(c);
+    for (int kind = 1; kind <= 2; ++kind) {
+        if (!(what & kind)) continue;
+        for (size_t k = 0; k < c->classes.size(); ++k) {
+            ClassWork& cw = c->classes[k];
+            LaunchShape best = kind == 1 ? cw.jk_shape : cw.grad_shape;
+            float best_ms = 1e30f;
+            for (int b : blocks)
+                for (int g : grids) {
+                    const LaunchShape sh{g, b};
+                    float total = 0.0f;
+                    for (int rep = 0; rep < 2; ++rep) {
+                        if ((r = hfg::run_one_class(c, k, kind == 2, sh))) return r;
+                        float ms = 0.0f;
+                        HFG_CHECK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+                        if (rep > 0) total += ms;  // the first run warms caches and code
+                    }
+                    if (total < best_ms) {
+                        best_ms = total;
+                        best = sh;
+                    }
+                }
+            (kind == 1 ? cw.jk_shape : cw.grad_shape) = best;
+        }
+    }
+    return HFG_OK;
+}
+
+int hfg_launch_shapes(const hfg_ctx* c, int cap, int* la_lb_lc_ld, int* jk_grid, int* jk_block, int* grad_grid,
+                      int* grad_block) {
+    if (!c) return HFG_ERR_ARG;
+    const int n = (int)c->classes.size();
+    for (int k = 0; k < n && k < cap; ++k) {
+        const ClassWork& cw = c->classes[k];
+        if (la_lb_lc_ld) la_lb_lc_ld[k] = cw.la * 1000 + cw.lb * 100 + cw.lc * 10 + cw.ld;
+        if (jk_grid) jk_grid[k] = cw.jk_shape.grid;
+        if (jk_block) jk_block[k] = cw.jk_shape.block;
+        if (grad_grid) grad_grid[k] = cw.grad_shape.grid;
+        if (grad_block) grad_block[k] = cw.grad_shape.block;
+    }
+    return n;
+}
+
+int hfg_class_stats(hfg_ctx* c, int cap, int* la_lb_lc_ld, long long* quartets, double* model_flops,
+                    double* kernel_ms) {
+    if (!c) return HFG_ERR_ARG;
+    const int n = (int)c->classes.size();
+    if (cap > 0) HFG_CHECK(c, hipStreamSynchronize(c->stream));
+    for (int k = 0; k < n && k < cap; ++k) {
+        ClassWork& cw = c->classes[k];
+        if (la_lb_lc_ld) la_lb_lc_ld[k] = cw.la * 1000 + cw.lb * 100 + cw.lc * 10 + cw.ld;
+        if (quartets) quartets[k] = cw.quartets;
+        if (model_flops) model_flops[k] = cw.flops;
+        if (kernel_ms) {
+            float ms = -1.0f;
+            if (c->class_timed && cw.ev_start && cw.ev_stop) HFG_CHECK(c, hipEventElapsedTime(&ms, cw.ev_start, cw.ev_stop));
+            kernel_ms[k] = ms;
+        }
+    }
+    return n;
+}
+
 int hfg_energy(hfg_ctx* c, const double* D, double* E2) {
     int r = check_ready(c);
     if (r) return r;
     if (!D || !E2) return fail(c, HFG_ERR_ARG, "hfg_energy: NULL argument");
-    const size_t nn = (size_t)c->nbf * c->nbf;
+    const size_t nn = (size_t)c->nn_total;
     std::vector<double> J(nn), K(nn);
     if ((r = hfg_build_jk(c, D, J.data(), K.data(), 0))) return r;
     double e = 0;
@@ -710,6 +1069,7 @@ int hfg_energy_backward(hfg_ctx* c, const double* D, double* grad) {
 int hfg_eri_shell_quartet(hfg_ctx* c, int s1, int s2, int s3, int s4, double* block) {
     int r = check_ready(c);
     if (r) return r;
+    if (c->mols.size() != 1) return fail(c, HFG_ERR_ARG, "hfg_eri_shell_quartet: single-molecule contexts only");
     if (!block || s1 < 0 || s2 < 0 || s3 < 0 || s4 < 0 || s1 >= c->nshell || s2 >= c->nshell || s3 >= c->nshell ||
         s4 >= c->nshell)
         return fail(c, HFG_ERR_ARG, "hfg_eri_shell_quartet: bad shell index");
@@ -741,6 +1101,7 @@ int hfg_schwarz(hfg_ctx* c, double* screen) {
     int r = check_ready(c);
     if (r) return r;
     if (!screen) return fail(c, HFG_ERR_ARG, "hfg_schwarz: NULL argument");
+    if (c->mols.size() != 1) return fail(c, HFG_ERR_ARG, "hfg_schwarz: single-molecule contexts only");
     std::memcpy(screen, c->screen.data(), c->screen.size() * sizeof(double));
     return HFG_OK;
 }
@@ -759,6 +1120,11 @@ void hfg_destroy(hfg_ctx* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    destroy_graphs(c);
+    for (auto& cw : c->classes) {
+        if (cw.ev_start) (void)hipEventDestroy(cw.ev_start);
+        if (cw.ev_stop) (void)hipEventDestroy(cw.ev_stop);
+    }
     c->d_boys.release();
     c->d_D.release();
     c->d_J.release();
@@ -776,6 +1142,7 @@ void hfg_destroy(hfg_ctx* c) {
     c->d_pp1.release();
     c->d_pp2.release();
     c->d_pairs.release();
+    c->d_mols.release();
     c->d_bp_off.release();
     c->d_bp_cnt.release();
     c->d_blk_nf.release();
@@ -784,6 +1151,11 @@ void hfg_destroy(hfg_ctx* c) {
     c->d_gJ_off.release();
     c->d_gK_off.release();
     c->d_shell_atom.release();
+    c->d_queue.release();
+    c->d_mol_items.release();
+    c->d_mol_items_off.release();
+    c->d_out_mol.release();
+    c->d_out_k.release();
     c->d_items.release();
     c->d_tiles.release();
     c->d_gJ.release();

@@ -48,6 +48,29 @@ struct ClassWork {
     long long quartets = 0;
     double flops = 0;
     double grad_flops = 0;
+    LaunchShape jk_shape{0, 0};    // persistent workgroups / block of the J/K launch (0 = automatic)
+    LaunchShape grad_shape{0, 0};  // the same for the gradient launch
+    hipEvent_t ev_start = nullptr, ev_stop = nullptr;  // brackets of the last direct launch
+};
+
+// One molecule of a context (a context holds one, or many in batching mode).
+struct MolHost {
+    int shell0 = 0, nshell = 0;  // global shell range
+    int atom0 = 0, natom = 0;    // global atom range (geometry)
+    int nbf = 0, ncart = 0;
+    int blk0 = 0, nblk = 0;      // global block range
+    int bp0 = 0, nbp = 0;        // global block-pair range
+    long long moff = 0;          // offset of its n x n matrices in concatenated arrays
+    int dm_off = 0;              // offset of its nshell x nshell density-maximum table
+    int gout0 = 0;               // offset of its natom x 3 gradient in concatenated output
+};
+
+// Cached graph of one J/K build configuration.
+struct JkGraph {
+    bool valid = false;
+    int mixed = 0;
+    bool dscreen = false;
+    hipGraphExec_t exec = nullptr;
 };
 
 }  // namespace hfg
@@ -68,32 +91,38 @@ struct hfg_ctx {
     hfg_options opt{};
     std::string err;
     bool have_basis = false, have_geom = false;
-    // basis
-    int nshell = 0, nbf = 0, ncart = 0;
+    bool batch = false;  // molecules added with hfg_batch_add_molecule
+    // molecules
+    std::vector<hfg::MolHost> mols;
+    long long nn_total = 0;  // sum of n^2 over molecules
+    int nout_total = 0;      // sum of 3 natom over molecules
+    // basis (concatenated over molecules; offsets within each molecule)
+    int nshell = 0, nbf = 0, ncart = 0;  // nbf / ncart: of molecule 0 (single-molecule calls)
     bool pure = false;
-    std::vector<int> L, atom, nprim, prim_off, cart_off, pure_off;
+    std::vector<int> L, atom, atom_local, nprim, prim_off, cart_off, pure_off, shell_mol;
     std::vector<double> alpha, coef;
-    // geometry
+    // geometry (concatenated)
     int natom = 0;
     std::vector<double> xyz;
     // blocks
     int nblk = 0;
-    std::vector<int> blk_L, blk_nf, blk_func_off, blk_func_ao, shell_blk, shell_slot;
+    std::vector<int> blk_L, blk_nf, blk_func_off, blk_func_ao, blk_mol, shell_blk, shell_slot;
     std::vector<std::vector<int>> blk_shells;
+    // block pairs (X >= Y, same molecule)
+    std::vector<int> bp_X, bp_Y;
     // pairs
     std::vector<PairDev> pairs;
     std::vector<double> psmean;
     std::vector<double> psq;  // sqrt|(pq|pq)| per pair component
     std::vector<int> bp_off, bp_cnt;
-    std::vector<double4> pp0, pp1, pp2;
-    std::vector<double> screen;
+    std::vector<double> screen;  // Cartesian (pq|pq) of molecule 0 (single-molecule contexts)
     double tol = 0.0;
     // work
     std::vector<ClassWork> classes;
-    std::vector<int2> tiles;
+    std::vector<int4> tiles;
     std::vector<int> gJ_off, gK_off;
     std::vector<GatherEntry> gJ, gK;
-    long long slab_doubles = 0, total_quartets = 0, total_items = 0;
+    long long slab_doubles = 0, grad_slab_doubles = 0, total_quartets = 0, total_items = 0;
     double total_flops = 0, total_grad_flops = 0;
     // device
     DevBuf<double> d_psq, d_boys, d_D, d_J, d_K, d_F, d_H, d_G, d_slab, d_psmean, d_grad_slab, d_grad;
@@ -101,10 +130,16 @@ struct hfg_ctx {
     std::vector<float> dmax;
     DevBuf<double4> d_pp0, d_pp1, d_pp2;
     DevBuf<PairDev> d_pairs;
+    DevBuf<hfg::MolDev> d_mols;
     DevBuf<int> d_bp_off, d_bp_cnt, d_blk_nf, d_blk_func_off, d_blk_func_ao, d_gJ_off, d_gK_off, d_shell_atom;
+    DevBuf<int> d_queue;  // one work-item counter per class launch (J/K, then gradient)
+    DevBuf<int> d_mol_items, d_mol_items_off, d_out_mol, d_out_k;
     DevBuf<WorkItem> d_items;
-    DevBuf<int2> d_tiles;
+    DevBuf<int4> d_tiles;
     DevBuf<GatherEntry> d_gJ, d_gK;
+    // launch graphs of the J/K build (keyed by precision and density screening)
+    std::vector<hfg::JkGraph> graphs;
+    bool class_timed = false;  // the last J/K build bracketed each class launch with events
     // incremental Fock state
     bool have_prev = false;
     std::vector<double> Dprev;
@@ -122,16 +157,18 @@ struct hfg_ctx {
 
 
 namespace hfg {
-// Two-electron gradient (natom x 3) through the adjoint derivative kernels.
+// Two-electron gradients (concatenated natom x 3 per molecule) through the adjoint
+// derivative kernels, for the concatenated densities D.
 int run_gradient(hfg_ctx* c, const double* D, double* grad);
 }  // namespace hfg
 
 namespace hfg {
-// Launch fn(class_work, stream) for every class on the side streams (fork/join with the main
-// stream): classes are independent until the gather.
+// Launch fn(class_work, stream) for every class on the side streams (fork from and join back
+// to `main`): classes are independent until the gather. Under stream capture the fork/join
+// events make the side-stream launches part of the captured graph.
 template <typename F>
-inline hipError_t for_each_class_concurrent(hfg_ctx* c, F&& fn) {
-    hipError_t e = hipEventRecord(c->ev_fork, c->stream);
+inline hipError_t for_each_class_concurrent(hfg_ctx* c, hipStream_t main, F&& fn) {
+    hipError_t e = hipEventRecord(c->ev_fork, main);
     if (e != hipSuccess) return e;
     for (int k = 0; k < kSideStreams; ++k)
         if ((e = hipStreamWaitEvent(c->side[k], c->ev_fork, 0)) != hipSuccess) return e;
@@ -139,8 +176,12 @@ inline hipError_t for_each_class_concurrent(hfg_ctx* c, F&& fn) {
         if ((e = fn(c->classes[i], c->side[c->class_stream[i]])) != hipSuccess) return e;
     for (int k = 0; k < kSideStreams; ++k) {
         if ((e = hipEventRecord(c->ev_join[k], c->side[k])) != hipSuccess) return e;
-        if ((e = hipStreamWaitEvent(c->stream, c->ev_join[k], 0)) != hipSuccess) return e;
+        if ((e = hipStreamWaitEvent(main, c->ev_join[k], 0)) != hipSuccess) return e;
     }
     return hipSuccess;
 }
+
+// Run class k alone (J/K, or the gradient when grad) with launch shape sh on the main stream,
+// bracketed by ev0 / ev1, and wait for it (the autotuner's measurement).
+int run_one_class(hfg_ctx* c, size_t k, bool grad, LaunchShape sh);
 }  // namespace hfg

@@ -10,11 +10,13 @@
 // the HRR to the six shifted classes and contracts them with G at once, so derivative
 // integrals are never stored.
 #pragma once
-#include "jk_impl.hpp"
+#include "boys.hpp"
+#include "common.hpp"
+#include "hgp.hpp"
 
 namespace hfg {
 
-constexpr int kMaxAtoms = 128;
+constexpr int kMaxAtoms = kMaxAtomsPerMolecule;
 
 struct GradArgs {
     const WorkItem* items;
@@ -27,14 +29,13 @@ struct GradArgs {
     const double4* pp1;
     const double4* pp2;
     const double* boys;
-    const double* D;
-    const int* shell_atom;
-    int n;
-    int natom;
+    const double* D;         // concatenated densities (MolDev::moff, MolDev::n)
+    const MolDev* mols;
+    const int* shell_atom;   // global shell -> atom within its molecule
     double tol;
     double prim_cut;
-    double* slab;  // natom*3 per work item (indexed by item position)
-    int item_base;
+    double* slab;            // 3 * natom(mol) partials per work item at WorkItem::gslab
+    int* queue;              // work-item counter of this launch (zeroed before the launch)
 };
 
 // Spherical -> Cartesian back-transform of one index: out[c] = sum_s Y(c, s) in[s].
@@ -151,10 +152,18 @@ __global__ __launch_bounds__(256) void grad_kernel(GradArgs a) {
     constexpr int LT = LA + LB + LC + LD;
     __shared__ double gsum[kMaxAtoms * 3];
     __shared__ int rowpre[kMaxRows + 1];
-    for (int item = blockIdx.x; item < a.nitems; item += gridDim.x) {
+    __shared__ int next_item;
+    for (;;) {
+    __syncthreads();  // the previous item's flush has read gsum and next_item
+    if (threadIdx.x == 0) next_item = atomicAdd(a.queue, 1);
+    __syncthreads();
+    const int item = next_item;
+    if (item >= a.nitems) break;
     const WorkItem w = a.items[item];
-    __syncthreads();  // the previous item's flush has read gsum
-    for (int k = threadIdx.x; k < a.natom * 3; k += blockDim.x) gsum[k] = 0.0;
+    const MolDev md = a.mols[w.mol];
+    const double* __restrict__ Dm = a.D + md.moff;
+    const int natom3 = md.natom * 3;
+    for (int k = threadIdx.x; k < natom3; k += blockDim.x) gsum[k] = 0.0;
     const int nrows = w.r1 - w.r0;
     const int bpo = a.bp_off[w.P], kpo = a.bp_off[w.Q], kn = a.bp_cnt[w.Q];
     const bool same = (w.P == w.Q);
@@ -193,17 +202,17 @@ __global__ __launch_bounds__(256) void grad_kernel(GradArgs a) {
         {
             R Gs[NA * NB * NC * ND];
             const int oa = bp.ao_a, ob = bp.ao_b, oc = kp.ao_a, od = kp.ao_b;
-            const int n = a.n;
+            const int n = md.n;
             sfor<NA>([&](auto p) {
                 sfor<NB>([&](auto q) {
-                    const R dpq = a.D[(size_t)(oa + p) * n + ob + q];
+                    const R dpq = Dm[(size_t)(oa + p) * n + ob + q];
                     sfor<NC>([&](auto r) {
-                        const R dpr = a.D[(size_t)(oa + p) * n + oc + r];
-                        const R dqr = a.D[(size_t)(ob + q) * n + oc + r];
+                        const R dpr = Dm[(size_t)(oa + p) * n + oc + r];
+                        const R dqr = Dm[(size_t)(ob + q) * n + oc + r];
                         sfor<ND>([&](auto s) {
-                            const R drs = a.D[(size_t)(oc + r) * n + od + s];
-                            const R dqs = a.D[(size_t)(ob + q) * n + od + s];
-                            const R dps = a.D[(size_t)(oa + p) * n + od + s];
+                            const R drs = Dm[(size_t)(oc + r) * n + od + s];
+                            const R dqs = Dm[(size_t)(ob + q) * n + od + s];
+                            const R dps = Dm[(size_t)(oa + p) * n + od + s];
                             Gs[((p * NB + q) * NC + r) * ND + s] =
                                 (R)deg * ((R)2 * dpq * drs - (R)0.5 * (dpr * dqs + dps * dqr));
                         });
@@ -216,14 +225,16 @@ __global__ __launch_bounds__(256) void grad_kernel(GradArgs a) {
         const R AB[3] = {(R)bp.AB[0], (R)bp.AB[1], (R)bp.AB[2]};
         const R CD[3] = {(R)kp.AB[0], (R)kp.AB[1], (R)kp.AB[2]};
         for (int ip = 0; ip < bp.np; ++ip) {
-            const double4 a0 = a.pp0[bp.off + ip];
-            const double4 a1 = a.pp1[bp.off + ip];
-            const double4 a2 = a.pp2[bp.off + ip];
+            const int ia = bp.off + ip * bp.pstride;
+            const double4 a0 = a.pp0[ia];
+            const double4 a1 = a.pp1[ia];
+            const double4 a2 = a.pp2[ia];
             const double p = a0.x;
             for (int jp = 0; jp < kp.np; ++jp) {
-                const double4 b0 = a.pp0[kp.off + jp];
-                const double4 b1 = a.pp1[kp.off + jp];
-                const double4 b2 = a.pp2[kp.off + jp];
+                const int ib = kp.off + jp * kp.pstride;
+                const double4 b0 = a.pp0[ib];
+                const double4 b1 = a.pp1[ib];
+                const double4 b2 = a.pp2[ib];
                 const double q = b0.x;
                 const double rs = rsqrt(p + q);
                 const double pfac = a1.w * b1.w * rs;
@@ -281,8 +292,9 @@ __global__ __launch_bounds__(256) void grad_kernel(GradArgs a) {
                 }
             }
         }
-        const int atA = a.shell_atom[bp.sa], atB = a.shell_atom[bp.sb];
-        const int atC = a.shell_atom[kp.sa], atD = a.shell_atom[kp.sb];
+        const int* sat = a.shell_atom + md.shell0;
+        const int atA = sat[bp.sa], atB = sat[bp.sb];
+        const int atC = sat[kp.sa], atD = sat[kp.sb];
         for (int d = 0; d < 3; ++d) {
             const double xa = gA[d], xb = gB[d], xc = gC[d];
             atomicAdd(&gsum[atA * 3 + d], xa);
@@ -292,18 +304,29 @@ __global__ __launch_bounds__(256) void grad_kernel(GradArgs a) {
         }
     }
     __syncthreads();
-    double* out = a.slab + (size_t)(a.item_base + item) * a.natom * 3;
-    for (int k = threadIdx.x; k < a.natom * 3; k += blockDim.x) out[k] = gsum[k];
+    double* out = a.slab + w.gslab;
+    for (int k = threadIdx.x; k < natom3; k += blockDim.x) out[k] = gsum[k];
     }
 }
 
-template <int LA, int LB, int LC, int LD>
-hipError_t launch_grad_class64(bool pure, const GradArgs& a, int nitems, hipStream_t s);
-template <int LA, int LB, int LC, int LD>
-hipError_t launch_grad_class32(bool pure, const GradArgs& a, int nitems, hipStream_t s);
+template <int LA, int LB, int LC, int LD, bool PURE>
+hipError_t launch_grad_class64(const GradArgs& a, int nitems, LaunchShape sh, hipStream_t s);
+template <int LA, int LB, int LC, int LD, bool PURE>
+hipError_t launch_grad_class32(const GradArgs& a, int nitems, LaunchShape sh, hipStream_t s);
 
 hipError_t launch_grad(int la, int lb, int lc, int ld, bool pure, int mixed, const GradArgs& a, int nitems,
-                       hipStream_t s);
-hipError_t launch_grad_reduce(const double* slab, int nitems, int nout, double* out, hipStream_t s);
+                       LaunchShape sh, hipStream_t s);
+
+// Per-molecule gradient sums over work items (fixed order).
+struct GradReduceArgs {
+    const double* slab;
+    const WorkItem* items;     // all work items (gslab offsets)
+    const int* mol_items;      // CSR: item indices of each molecule
+    const int* mol_items_off;
+    const int* out_mol;        // molecule of each output component
+    const int* out_k;          // component within the molecule (atom * 3 + direction)
+    double* out;               // concatenated natom(mol) x 3 gradients
+};
+hipError_t launch_grad_reduce(const GradReduceArgs& a, int nout, hipStream_t s);
 
 }  // namespace hfg

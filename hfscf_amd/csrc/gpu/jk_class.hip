@@ -1,6 +1,6 @@
-// One angular-momentum class of the J/K kernel; built once per class and precision with
-// -DHFG_LA=.. -DHFG_LB=.. -DHFG_LC=.. -DHFG_LD=.. [-DHFG_MIXED=1] so the classes compile in
-// parallel. The mixed build evaluates and digests the quartets in FP32 (Boys values, pair
+// One angular-momentum class of the J/K kernel; built once per class, basis type and precision
+// with -DHFG_LA=.. -DHFG_LB=.. -DHFG_LC=.. -DHFG_LD=.. -DHFG_PURE=0|1 [-DHFG_MIXED=1] so the
+// variants compile in parallel. The mixed build evaluates and digests the quartets in FP32 (Boys values, pair
 // data and screening in FP64) and accumulates the LDS tiles, the slabs and the gather in FP64.
 #include "jk_impl.hpp"
 
@@ -19,17 +19,15 @@ using Real = double;
 #endif
 
 template <>
-hipError_t HFG_LAUNCH<HFG_LA, HFG_LB, HFG_LC, HFG_LD>(bool pure, const JkArgs& a, int nitems, hipStream_t s) {
+hipError_t HFG_LAUNCH<HFG_LA, HFG_LB, HFG_LC, HFG_LD, (HFG_PURE != 0)>(const JkArgs& a, int nitems, LaunchShape sh,
+                                                                     hipStream_t s) {
     JkArgs b = a;
     b.nitems = nitems;
-    const int block = 256;
-    if (pure) {
-        auto k = jk_kernel<HFG_LA, HFG_LB, HFG_LC, HFG_LD, true, Real, double>;
-        hipLaunchKernelGGL(k, dim3(grid_for(k, nitems, block)), dim3(block), 0, s, b);
-    } else {
-        auto k = jk_kernel<HFG_LA, HFG_LB, HFG_LC, HFG_LD, false, Real, double>;
-        hipLaunchKernelGGL(k, dim3(grid_for(k, nitems, block)), dim3(block), 0, s, b);
-    }
+    constexpr int LT = HFG_LA + HFG_LB + HFG_LC + HFG_LD;
+    constexpr size_t est = sizeof(Real) * vrr_entries(HFG_LA + HFG_LB, HFG_LC + HFG_LD, LT);
+    auto k = jk_kernel<HFG_LA, HFG_LB, HFG_LC, HFG_LD, (HFG_PURE != 0), Real, double>;
+    const LaunchShape ls = bounded_shape(k, nitems, sh, est);
+    hipLaunchKernelGGL(k, dim3(ls.grid), dim3(ls.block), 0, s, b);
     return hipGetLastError();
 }
 

@@ -1,23 +1,29 @@
-// Run-time dispatch from (la lb|lc ld) and precision to the specialised J/K kernel.
+// Run-time dispatch from (la lb|lc ld), basis type and precision to the specialised J/K kernel.
 #include "jk_impl.hpp"
 
 namespace hfg {
 
-#define HFG_DECL(A, B, C, D)                                                                       \
-    template <>                                                                                    \
-    hipError_t launch_jk_class64<A, B, C, D>(bool, const JkArgs&, int, hipStream_t);               \
-    template <>                                                                                    \
-    hipError_t launch_jk_class32<A, B, C, D>(bool, const JkArgs&, int, hipStream_t);
+#define HFG_DECL1(A, B, C, D, P)                                                                  \
+    template <>                                                                                   \
+    hipError_t launch_jk_class64<A, B, C, D, P>(const JkArgs&, int, LaunchShape, hipStream_t);    \
+    template <>                                                                                   \
+    hipError_t launch_jk_class32<A, B, C, D, P>(const JkArgs&, int, LaunchShape, hipStream_t);
+#define HFG_DECL(A, B, C, D) HFG_DECL1(A, B, C, D, true) HFG_DECL1(A, B, C, D, false)
 HFG_FOR_CLASSES(HFG_DECL)
 #undef HFG_DECL
+#undef HFG_DECL1
 
 hipError_t launch_jk(int la, int lb, int lc, int ld, bool pure, int mixed, const JkArgs& a, int nitems,
-                     hipStream_t s) {
+                     LaunchShape sh, hipStream_t s) {
     if (nitems == 0) return hipSuccess;
-#define HFG_CASE(A, B, C, D)                                                                     \
-    if (la == A && lb == B && lc == C && ld == D)                                                \
-        return mixed ? launch_jk_class32<A, B, C, D>(pure, a, nitems, s)                         \
-                     : launch_jk_class64<A, B, C, D>(pure, a, nitems, s);
+#define HFG_CASE(A, B, C, D)                                                                      \
+    if (la == A && lb == B && lc == C && ld == D) {                                               \
+        if (pure)                                                                                 \
+            return mixed ? launch_jk_class32<A, B, C, D, true>(a, nitems, sh, s)                  \
+                         : launch_jk_class64<A, B, C, D, true>(a, nitems, sh, s);                 \
+        return mixed ? launch_jk_class32<A, B, C, D, false>(a, nitems, sh, s)                     \
+                     : launch_jk_class64<A, B, C, D, false>(a, nitems, sh, s);                    \
+    }
     HFG_FOR_CLASSES(HFG_CASE)
 #undef HFG_CASE
     return hipErrorInvalidValue;

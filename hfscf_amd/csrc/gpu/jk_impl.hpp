@@ -1,38 +1,9 @@
 // J/K work-item kernel template (one instantiation per angular-momentum class).
 #pragma once
-#include "kernels.hpp"
-#include <algorithm>
-
+#include "common.hpp"
 #include "quartet.hpp"
 
 namespace hfg {
-
-// Number of ket pairs j (descending mean Schwarz list) with s_i * s_j >= tol.
-HFG_INLINE int ket_count(const double* __restrict__ s, int n, double si, double tol) {
-    int lo = 0, hi = n;
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (s[mid] * si >= tol)
-            lo = mid + 1;
-        else
-            hi = mid;
-    }
-    return lo;
-}
-
-// Quartet t of a work item (flattened over its rows) -> (bra row i, ket column j).
-HFG_INLINE void locate(int t, const int* rowpre, int nrows, const WorkItem& w, int& i, int& j) {
-    int lo = 0, hi = nrows;  // largest r with rowpre[r] <= t
-    while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (rowpre[mid] <= t)
-            lo = mid;
-        else
-            hi = mid;
-    }
-    i = w.r0 + lo;
-    j = w.c0 + t - rowpre[lo];
-}
 
 // Density-weighted Schwarz bound of a quartet, the form of hfscf's (disabled) direct digest
 // (hfscf: src/scfsolver/hfscf_hf_solver.cpp:317-324) on shell blocks.
@@ -56,9 +27,23 @@ __global__ __launch_bounds__(256) void jk_kernel(JkArgs a) {
     __shared__ int rowpre[kMaxRows + 1];
     __shared__ int clist[kChunk];
     __shared__ int ccount;
+    __shared__ int next_item;
     const bool dscreen = a.dtol > 0.0;
-    for (int item = blockIdx.x; item < a.nitems; item += gridDim.x) {
+    constexpr bool kDeferJab = NA * NB <= 9;  // s/p bra pairs: J_ab partials in registers
+    R jab[kDeferJab ? NA * NB : 1];
+    int jab_row = -1, jab_o = 0;  // bra row of the partials, LDS offset of its J_ab block
+    // persistent workgroups: each pulls work items from the launch's device queue until it is
+    // drained (the counter only grows, so every workgroup leaves the loop)
+    for (;;) {
+        __syncthreads();  // the previous item's flush has read the tiles and next_item
+        if (threadIdx.x == 0) next_item = atomicAdd(a.queue, 1);
+        __syncthreads();
+        const int item = next_item;
+        if (item >= a.nitems) break;
         const WorkItem w = a.items[item];
+        const MolDev md = a.mols[w.mol];
+        const double* __restrict__ Dm = a.D + md.moff;
+        const float* __restrict__ dm = a.dmax + md.dm_off;
         const int fA = a.blk_nf[w.A], fB = a.blk_nf[w.B], fC = a.blk_nf[w.C], fD = a.blk_nf[w.D];
         ACC* tJab = tiles;
         ACC* tJcd = tJab + fA * fB;
@@ -67,7 +52,6 @@ __global__ __launch_bounds__(256) void jk_kernel(JkArgs a) {
         ACC* tKbc = tKad + fA * fD;
         ACC* tKbd = tKbc + fB * fC;
         const int ntile = fA * fB + fC * fD + fA * fC + fA * fD + fB * fC + fB * fD;
-        __syncthreads();  // the previous item's flush has read the tiles
         for (int k = threadIdx.x; k < ntile; k += blockDim.x) tiles[k] = (ACC)0;
         const int nrows = w.r1 - w.r0;
         const int bpo = a.bp_off[w.P], kpo = a.bp_off[w.Q], kn = a.bp_cnt[w.Q];
@@ -85,6 +69,15 @@ __global__ __launch_bounds__(256) void jk_kernel(JkArgs a) {
         }
         __syncthreads();
         const int total = rowpre[nrows];
+        jab_row = -1;
+        auto flush_jab = [&]() {
+            if constexpr (kDeferJab) {
+                if (jab_row >= 0)
+                    sfor<NA>([&](auto p) {
+                        sfor<NB>([&](auto q) { atomicAdd(&tJab[jab_o + p * fB + q], (ACC)jab[p * NB + q]); });
+                    });
+            }
+        };
         // total and dscreen are uniform over the workgroup, so every thread takes the same
         // number of passes and reaches the same barriers
         for (int base = 0; base < total; base += kChunk) {
@@ -96,7 +89,7 @@ __global__ __launch_bounds__(256) void jk_kernel(JkArgs a) {
                 for (int u = threadIdx.x; u < cnt; u += blockDim.x) {
                     int i, j;
                     locate(base + u, rowpre, nrows, w, i, j);
-                    if (density_bound(a.pairs[bpo + i], a.pairs[kpo + j], a.dmax, a.nshell) >= a.dtol)
+                    if (density_bound(a.pairs[bpo + i], a.pairs[kpo + j], dm, md.nshell) >= a.dtol)
                         clist[atomicAdd(&ccount, 1)] = base + u;
                 }
                 __syncthreads();
@@ -114,11 +107,20 @@ __global__ __launch_bounds__(256) void jk_kernel(JkArgs a) {
                 R I[NA * NB * NC * ND];
                 eri_block<LA, LB, LC, LD, PURE, R>(bp, kp, a.pp0, a.pp1, a.pp2, a.boys, a.prim_cut, a.psq, a.elem_tol,
                                                    I);
-                digest<NA, NB, NC, ND, R, ACC>(I, s, a.D, a.n, bp, kp, tJab, tJcd, tKac, tKad, tKbc, tKbd, fB, fC,
-                                               fD);
+                if constexpr (kDeferJab) {
+                    if (i != jab_row) {
+                        flush_jab();
+                        jab_row = i;
+                        jab_o = bp.slot_a * NA * fB + bp.slot_b * NB;
+                        sfor<NA * NB>([&](auto k) { jab[k] = (R)0; });
+                    }
+                }
+                digest<NA, NB, NC, ND, R, ACC, kDeferJab>(I, s, Dm, md.n, bp, kp, tJab, tJcd, tKac, tKad, tKbc, tKbd,
+                                                          fB, fC, fD, jab);
             }
             if (dscreen) __syncthreads();  // every thread has read ccount and clist
         }
+        flush_jab();
         __syncthreads();
         double* out = a.slab + w.slab;
         for (int k = threadIdx.x; k < ntile; k += blockDim.x) out[k] = (double)tiles[k];
@@ -126,49 +128,9 @@ __global__ __launch_bounds__(256) void jk_kernel(JkArgs a) {
 }
 
 // Launchers of one class, FP64 and mixed (each specialised in its own translation unit).
-template <int LA, int LB, int LC, int LD>
-hipError_t launch_jk_class64(bool pure, const JkArgs& a, int nitems, hipStream_t s);
-template <int LA, int LB, int LC, int LD>
-hipError_t launch_jk_class32(bool pure, const JkArgs& a, int nitems, hipStream_t s);
-
-// Workgroups for a launch of `kernel` over nitems work items: one per item, unless the
-// kernel's scratch would then exceed kScratchBudget for one dispatch.
-template <typename K>
-inline int grid_for(K kernel, int nitems, int block) {
-    hipFuncAttributes at{};
-    if (hipFuncGetAttributes(&at, reinterpret_cast<const void*>(kernel)) != hipSuccess) return nitems;
-    const double per_wg = (double)at.localSizeBytes * block;
-    if (per_wg <= 0.0) return nitems;
-    const int cap = (int)(kScratchBudget / per_wg);
-    return std::max(1, std::min(nitems, cap));
-}
-
-// Every class with la >= lb, la >= lc >= ld, l <= 2.
-#define HFG_FOR_CLASSES(X) \
-    X(0, 0, 0, 0)          \
-    X(1, 0, 0, 0)          \
-    X(1, 0, 1, 0)          \
-    X(1, 0, 1, 1)          \
-    X(1, 1, 0, 0)          \
-    X(1, 1, 1, 0)          \
-    X(1, 1, 1, 1)          \
-    X(2, 0, 0, 0)          \
-    X(2, 0, 1, 0)          \
-    X(2, 0, 1, 1)          \
-    X(2, 0, 2, 0)          \
-    X(2, 0, 2, 1)          \
-    X(2, 0, 2, 2)          \
-    X(2, 1, 0, 0)          \
-    X(2, 1, 1, 0)          \
-    X(2, 1, 1, 1)          \
-    X(2, 1, 2, 0)          \
-    X(2, 1, 2, 1)          \
-    X(2, 1, 2, 2)          \
-    X(2, 2, 0, 0)          \
-    X(2, 2, 1, 0)          \
-    X(2, 2, 1, 1)          \
-    X(2, 2, 2, 0)          \
-    X(2, 2, 2, 1)          \
-    X(2, 2, 2, 2)
+template <int LA, int LB, int LC, int LD, bool PURE>
+hipError_t launch_jk_class64(const JkArgs& a, int nitems, LaunchShape sh, hipStream_t s);
+template <int LA, int LB, int LC, int LD, bool PURE>
+hipError_t launch_jk_class32(const JkArgs& a, int nitems, LaunchShape sh, hipStream_t s);
 
 }  // namespace hfg

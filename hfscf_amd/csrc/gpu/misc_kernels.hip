@@ -1,4 +1,4 @@
-// Schwarz diagonal, gather and Fock kernels.
+// Schwarz diagonal, gather, counter reset and Fock kernels.
 #include "jk_impl.hpp"
 
 namespace hfg {
@@ -12,14 +12,13 @@ __global__ __launch_bounds__(64) void screen_kernel(ScreenArgs a) {
     constexpr int na = ncart(LA), nb = ncart(LB);
     const PairDev p = a.pairs[t];
     double blk[na * nb * na * nb];
-    eri_cart<LA, LB, LA, LB, double>(p, p, a.pp0, a.pp1, a.pp2, a.boys, 0.0, blk);
-    const int ca = a.cart_a[t], cb = a.cart_b[t];
+    // hfscf's screen matrix skips primitive quartets with |prefactor| < 1e-16
+    // (hfscf_oseri_screen.cpp:78); diagonals of distant pairs can vanish entirely, which then
+    // drops their quartets in the shell and element screens
+    eri_cart<LA, LB, LA, LB, double>(p, p, a.pp0, a.pp1, a.pp2, a.boys, 1e-16, blk);
+    double* out = a.out + a.out_off[t];
     for (int x = 0; x < na; ++x)
-        for (int y = 0; y < nb; ++y) {
-            const double v = blk[((x * nb + y) * na + x) * nb + y];
-            a.screen[(size_t)(ca + x) * a.ncart + cb + y] = v;
-            a.screen[(size_t)(cb + y) * a.ncart + ca + x] = v;
-        }
+        for (int y = 0; y < nb; ++y) out[x * nb + y] = blk[((x * nb + y) * na + x) * nb + y];
 }
 
 hipError_t launch_screen(int la, int lb, const ScreenArgs& a, hipStream_t s) {
@@ -45,7 +44,10 @@ hipError_t launch_screen(int la, int lb, const ScreenArgs& a, hipStream_t s) {
 // (transposed where the work item held it as (Y, X)) in a fixed order, then writes the
 // symmetric J and K elements.
 __global__ __launch_bounds__(256) void gather_kernel(GatherArgs a) {
-    const int2 xy = a.tiles[blockIdx.x];
+    const int4 xy = a.tiles[blockIdx.x];
+    const MolDev md = a.mols[xy.z];
+    double* __restrict__ J = a.J + md.moff;
+    double* __restrict__ K = a.K + md.moff;
     const int fX = a.blk_nf[xy.x], fY = a.blk_nf[xy.y];
     const int* aoX = a.blk_func_ao + a.blk_func_off[xy.x];
     const int* aoY = a.blk_func_ao + a.blk_func_off[xy.y];
@@ -63,18 +65,29 @@ __global__ __launch_bounds__(256) void gather_kernel(GatherArgs a) {
             const GatherEntry en = a.gK[g];
             sk += en.trans ? a.slab[en.off + (long long)v * en.cols + u] : a.slab[en.off + (long long)u * en.cols + v];
         }
-        const size_t i1 = (size_t)aoX[u] * a.n + aoY[v];
-        const size_t i2 = (size_t)aoY[v] * a.n + aoX[u];
-        a.J[i1] = sj;
-        a.J[i2] = sj;
-        a.K[i1] = sk;
-        a.K[i2] = sk;
+        const size_t i1 = (size_t)aoX[u] * md.n + aoY[v];
+        const size_t i2 = (size_t)aoY[v] * md.n + aoX[u];
+        J[i1] = sj;
+        J[i2] = sj;
+        K[i1] = sk;
+        K[i2] = sk;
     }
 }
 
 hipError_t launch_gather(const GatherArgs& a, int ntiles, hipStream_t s) {
     if (ntiles == 0) return hipSuccess;
     hipLaunchKernelGGL(gather_kernel, dim3(ntiles), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+__global__ void zero_ints_kernel(int* p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+hipError_t launch_zero_ints(int* p, int n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(zero_ints_kernel, dim3((n + 255) / 256), dim3(256), 0, s, p, n);
     return hipGetLastError();
 }
 

@@ -18,13 +18,15 @@ HFG_INLINE void eri_cart(const PairDev& bp, const PairDev& kp, const double4* __
     R acc[H::NE][H::NF];
     sfor<H::NE>([&](auto e) { sfor<H::NF>([&](auto f) { acc[e][f] = (R)0; }); });
     for (int i = 0; i < bp.np; ++i) {
-        const double4 a0 = pp0[bp.off + i];
-        const double4 a1 = pp1[bp.off + i];
-        const double o2p = pp2[bp.off + i].x;
+        const int ia = bp.off + i * bp.pstride;
+        const double4 a0 = pp0[ia];
+        const double4 a1 = pp1[ia];
+        const double o2p = pp2[ia].x;
         const double p = a0.x;
         for (int j = 0; j < kp.np; ++j) {
-            const double4 b0 = pp0[kp.off + j];
-            const double4 b1 = pp1[kp.off + j];
+            const int ib = kp.off + j * kp.pstride;
+            const double4 b0 = pp0[ib];
+            const double4 b1 = pp1[ib];
             const double q = b0.x;
             const double rs = rsqrt(p + q);
             const double pfac = a1.w * b1.w * rs;
@@ -48,7 +50,7 @@ HFG_INLINE void eri_cart(const PairDev& bp, const PairDev& kp, const double4* __
                 const R WP[3] = {(R)(-rp * PQx), (R)(-rp * PQy), (R)(-rp * PQz)};
                 const R WQ[3] = {(R)(rq * PQx), (R)(rq * PQy), (R)(rq * PQz)};
                 Vrr<LA + LB, LC + LD, R> V;
-                V.run(F, PA, WP, QC, WQ, (R)o2p, (R)rp, (R)pp2[kp.off + j].x, (R)rq, (R)(0.5 * ipq));
+                V.run(F, PA, WP, QC, WQ, (R)o2p, (R)rp, (R)pp2[ib].x, (R)rq, (R)(0.5 * ipq));
                 sfor<H::NE>([&](auto e) {
                     sfor<H::NF>([&](auto f) { acc[e][f] += V.v[H::NE0 + e][H::NF0 + f][0]; });
                 });
@@ -98,16 +100,28 @@ HFG_INLINE void eri_block(const PairDev& bp, const PairDev& kp, const double4* _
     } else {
         eri_cart<LA, LB, LC, LD, R>(bp, kp, pp0, pp1, pp2, boys_tab, prim_cut, out);
         element_screen<LA, LB, LC, LD, R>(bp, kp, sq, tol, out);
+        // with screening, hfscf stores the diagonal (pq|pq) of a Cartesian basis from its screen
+        // matrix (hfscf: src/integrals/hfscf_oseri.cpp:152-158,214-218)
+        if constexpr (LA == LC && LB == LD) {
+            if (tol > 0.0 && bp.sa == kp.sa && bp.sb == kp.sb) {
+                constexpr int nab = ncart(LA) * ncart(LB);
+                const double* s2 = sq + bp.sq_off;
+                sfor<nab>([&](auto ab) { out[ab * nab + ab] = (R)(s2[ab] * s2[ab]); });
+            }
+        }
     }
 }
 
 // Adds one quartet's contributions into the six LDS tiles. I is the output-basis block,
 // s the permutational weight (1/2 for each of a==b, c==d, ab==cd). J tiles receive
 // 2 s D I and K tiles s D I; the full matrices are J = T + T^T and K = T + T^T.
-template <int NA, int NB, int NC, int ND, typename R, typename ACC>
+// With DEFER_JAB the J_ab contribution is added to the caller's registers jab[NA*NB] instead of
+// the LDS tile: the lanes of a wavefront mostly share the bra pair, and their atomics on one
+// tile element would serialise.
+template <int NA, int NB, int NC, int ND, typename R, typename ACC, bool DEFER_JAB = false>
 HFG_INLINE void digest(const R* I, double s, const double* __restrict__ D, int n, const PairDev& bp,
                        const PairDev& kp, ACC* tJab, ACC* tJcd, ACC* tKac, ACC* tKad, ACC* tKbc, ACC* tKbd, int fB,
-                       int fC, int fD) {
+                       int fC, int fD, R* jab = nullptr) {
     const int ao_a = bp.ao_a, ao_b = bp.ao_b, ao_c = kp.ao_a, ao_d = kp.ao_b;
     const int la0 = bp.slot_a * NA, lb0 = bp.slot_b * NB, lc0 = kp.slot_a * NC, ld0 = kp.slot_b * ND;
     R dab[NA][NB], dcd[NC][ND], dac[NA][NC], dad[NA][ND], dbc[NB][NC], dbd[NB][ND];
@@ -127,7 +141,10 @@ HFG_INLINE void digest(const R* I, double s, const double* __restrict__ D, int n
         sfor<NB>([&](auto q) {
             R j = 0;
             sfor<NC>([&](auto r) { sfor<ND>([&](auto t) { j += dcd[r][t] * I[((p * NB + q) * NC + r) * ND + t]; }); });
-            atomicAdd(&tJab[(la0 + p) * fB + lb0 + q], (ACC)(sj * j));
+            if constexpr (DEFER_JAB)
+                jab[p * NB + q] += sj * j;
+            else
+                atomicAdd(&tJab[(la0 + p) * fB + lb0 + q], (ACC)(sj * j));
         });
         sfor<NC>([&](auto r) {
             R k = 0;

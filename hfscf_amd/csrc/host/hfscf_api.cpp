@@ -1,5 +1,7 @@
 // C ABI of the host RHF layer and the GPU two-electron backend (GpuJK) that plugs the
 // hfgpu library into the hf_solver seam.
+#include <algorithm>
+#include <chrono>
 #include <cstring>
 #include <memory>
 #include <stdexcept>
@@ -27,36 +29,46 @@ int set_err(const std::string& s) {
     return -1;
 }
 
+// Shell table of a molecule in the hfgpu layout.
+struct BasisArrays {
+    std::vector<int> L, atom, np, co, po;
+    std::vector<double> alpha, coef, xyz;
+    explicit BasisArrays(const Molecule& mol) {
+        for (const Shell& sh : mol.shells) {
+            L.push_back(sh.L);
+            atom.push_back(sh.atom);
+            np.push_back((int)sh.alpha.size());
+            co.push_back(sh.idx);
+            po.push_back(sh.ids);
+            alpha.insert(alpha.end(), sh.alpha.begin(), sh.alpha.end());
+            coef.insert(coef.end(), sh.c.begin(), sh.c.end());
+        }
+        for (auto& p : mol.xyz) xyz.insert(xyz.end(), p.begin(), p.end());
+    }
+};
+
+hfg_options gpu_options(const hfscf_scf_options& o) {
+    hfg_options go;
+    hfg_default_options(&go);
+    go.precision = o.precision;
+    go.integral_tol = o.integral_tol;
+    go.schwarz = o.schwarz;
+    go.density_tol = o.density_tol;
+    go.graph = o.graph;
+    return go;
+}
+
 // GPU backend: ERIOS/hf_solver/scf_gradient two-electron work through the hfgpu C ABI.
 class GpuJK final : public JKBuilder {
    public:
     GpuJK(const Molecule& mol, const hfscf_scf_options& o) : mol_(mol), precision_(o.precision) {
-        hfg_options go;
-        hfg_default_options(&go);
-        go.precision = o.precision;
-        go.integral_tol = o.integral_tol;
-        go.schwarz = o.schwarz;
-        go.density_tol = o.density_tol;
+        const hfg_options go = gpu_options(o);
         int r = hfg_create(&ctx_, o.device, &go);
         if (r) throw std::runtime_error("hfg_create failed (" + std::to_string(r) + "): no usable GPU");
-        const int ns = (int)mol.shells.size();
-        std::vector<int> L(ns), atom(ns), np(ns), co(ns), po(ns);
-        std::vector<double> a, c;
-        for (int s = 0; s < ns; ++s) {
-            const Shell& sh = mol.shells[s];
-            L[s] = sh.L;
-            atom[s] = sh.atom;
-            np[s] = (int)sh.alpha.size();
-            co[s] = sh.idx;
-            po[s] = sh.ids;
-            a.insert(a.end(), sh.alpha.begin(), sh.alpha.end());
-            c.insert(c.end(), sh.c.begin(), sh.c.end());
-        }
-        check(hfg_set_basis(ctx_, ns, L.data(), atom.data(), np.data(), a.data(), c.data(), mol.pure ? 1 : 0,
-                            co.data(), po.data()));
-        std::vector<double> xyz;
-        for (auto& p : mol.xyz) xyz.insert(xyz.end(), p.begin(), p.end());
-        check(hfg_set_geometry(ctx_, mol.natom(), xyz.data(), mol.Z.data()));
+        BasisArrays b(mol);
+        check(hfg_set_basis(ctx_, (int)b.L.size(), b.L.data(), b.atom.data(), b.np.data(), b.alpha.data(),
+                            b.coef.data(), mol.pure ? 1 : 0, b.co.data(), b.po.data()));
+        check(hfg_set_geometry(ctx_, mol.natom(), b.xyz.data(), mol.Z.data()));
     }
     ~GpuJK() override { hfg_destroy(ctx_); }
     void build_jk(const double* D, double* J, double* K) override { check(hfg_build_jk(ctx_, D, J, K, 0)); }
@@ -118,6 +130,168 @@ std::unique_ptr<JKBuilder> make_backend(const Molecule& mol, const hfscf_scf_opt
     }
     if (o.backend != HFSCF_BACKEND_GPU) throw std::runtime_error("unknown backend");
     return std::make_unique<GpuJK>(mol, o);
+}
+
+// Lockstep SCF of many molecules with one batched GPU Fock build per iteration.
+int batch_rhf(int nmol, hfscf_mol* const* mols, const hfscf_scf_options& opt, hfscf_scf_result* res, double* D_all,
+              double* F_all, double* grad_all, hfscf_batch_stats* st) {
+    using clk = std::chrono::steady_clock;
+    auto secs = [](clk::time_point a, clk::time_point b) { return std::chrono::duration<double>(b - a).count(); };
+    hfscf_batch_stats S{};
+    const auto t_setup = clk::now();
+    const hfg_options go = gpu_options(opt);
+    hfg_ctx* ctx = nullptr;
+    int r = hfg_create(&ctx, opt.device, &go);
+    if (r) throw std::runtime_error("hfg_create failed (" + std::to_string(r) + "): no usable GPU");
+    std::unique_ptr<hfg_ctx, void (*)(hfg_ctx*)> guard(ctx, hfg_destroy);
+    auto check = [&](int rc) {
+        if (rc < 0) throw std::runtime_error(std::string("hfgpu: ") + hfg_last_error(ctx));
+    };
+    for (int m = 0; m < nmol; ++m) {
+        const Molecule& mol = mols[m]->mol;
+        BasisArrays b(mol);
+        check(hfg_batch_add_molecule(ctx, (int)b.L.size(), b.L.data(), b.atom.data(), b.np.data(), b.alpha.data(),
+                                     b.coef.data(), mol.pure ? 1 : 0, b.co.data(), b.po.data(), mol.natom(),
+                                     b.xyz.data()));
+    }
+    check(hfg_batch_commit(ctx));
+    std::vector<long long> moff(nmol);
+    std::vector<int> nbf(nmol);
+    check(hfg_batch_layout(ctx, nullptr, nbf.data(), moff.data(), nullptr));
+    long long nn = 0;
+    for (int m = 0; m < nmol; ++m) nn = std::max(nn, moff[m] + (long long)nbf[m] * nbf[m]);
+    ScfOptions so;
+    so.diis_range = opt.diis_range;
+    so.energy_tol = opt.energy_tol;
+    so.rms_tol = opt.rms_tol;
+    so.max_iter = opt.max_iter;
+    so.incremental = opt.incremental != 0;
+    so.final_build = opt.final_fp64 != 0;
+    so.verbose = 0;
+    std::vector<std::unique_ptr<hf_solver>> sol(nmol);
+    std::vector<double> H(nn), D(nn), F(nn);
+    // one-electron set-up per molecule (independent, so in parallel)
+    std::string err;
+#pragma omp parallel for schedule(dynamic)
+    for (int m = 0; m < nmol; ++m) {
+        try {
+            sol[m] = std::make_unique<hf_solver>(mols[m]->mol, so, nullptr);
+            sol[m]->init_data();
+            sol[m]->begin();
+            std::copy(sol[m]->hcore().begin(), sol[m]->hcore().end(), H.begin() + moff[m]);
+        } catch (const std::exception& e) {
+#pragma omp critical
+            err = e.what();
+        }
+    }
+    if (!err.empty()) throw std::runtime_error(err);
+    S.setup_seconds = secs(t_setup, clk::now());
+    std::vector<char> active(nmol, 1), conv(nmol, 0);
+    const auto t_scf = clk::now();
+    for (int it = 1;; ++it) {
+        int nact = 0;
+        for (int m = 0; m < nmol; ++m) nact += active[m];
+        if (!nact) break;
+        for (int m = 0; m < nmol; ++m)
+            std::copy(sol[m]->density().begin(), sol[m]->density().end(), D.begin() + moff[m]);
+        // as GpuJK: the first (full) build in FP64, density-difference builds in the chosen precision
+        const int inc = (opt.incremental && it > 1) ? 1 : 0;
+        check(hfg_set_precision(ctx, (opt.precision == HFG_MIXED && inc) ? HFG_MIXED : HFG_FP64));
+        const auto t0 = clk::now();
+        check(hfg_fock(ctx, H.data(), D.data(), inc, F.data()));
+        const double tf = secs(t0, clk::now());
+        S.fock_seconds += tf;
+        S.iterations = it;
+        if (it == 1 && opt.autotune) {
+            const auto ta = clk::now();
+            check(hfg_autotune(ctx, 1));
+            S.autotune_seconds = secs(ta, clk::now());
+        }
+#pragma omp parallel for schedule(dynamic)
+        for (int m = 0; m < nmol; ++m) {
+            if (!active[m]) continue;
+            sol[m]->step(F.data() + moff[m], tf);
+            if (sol[m]->converged()) {
+                conv[m] = 1;
+                active[m] = 0;
+            } else if (sol[m]->iterations() >= so.max_iter) {
+                active[m] = 0;
+            }
+        }
+    }
+    // final full FP64 build at every converged density (energy_final)
+    std::vector<double> e_final(nmol);
+    for (int m = 0; m < nmol; ++m) e_final[m] = sol[m]->energy_electronic();
+    if (so.final_build) {
+        for (int m = 0; m < nmol; ++m)
+            std::copy(sol[m]->density().begin(), sol[m]->density().end(), D.begin() + moff[m]);
+        check(hfg_set_precision(ctx, HFG_FP64));
+        std::vector<double> Ff(nn);
+        check(hfg_fock(ctx, H.data(), D.data(), 0, Ff.data()));
+        for (int m = 0; m < nmol; ++m) {
+            double e = 0.0;
+            const size_t n2 = (size_t)nbf[m] * nbf[m];
+            for (size_t k = 0; k < n2; ++k) e += D[moff[m] + k] * (H[moff[m] + k] + Ff[moff[m] + k]);
+            e_final[m] = e;
+        }
+    }
+#pragma omp parallel for schedule(dynamic)
+    for (int m = 0; m < nmol; ++m) sol[m]->finish(e_final[m]);
+    S.scf_seconds = secs(t_scf, clk::now());
+    hfg_stats hs{};
+    hfg_last_stats(ctx, &hs);
+    S.work_items = (int)hs.work_items;
+    S.quartets = hs.quartets;
+    for (int m = 0; m < nmol; ++m) {
+        const hf_solver& s = *sol[m];
+        if (res) {
+            hfscf_scf_result& o = res[m];
+            o.energy = s.energy_total();
+            o.energy_elec = s.energy_electronic();
+            o.energy_nuc = mols[m]->mol.nuclear_repulsion();
+            o.iterations = s.iterations();
+            o.converged = conv[m];
+            o.fock_seconds = s.total_fock_seconds();
+            o.last_fock_seconds = s.last_fock_seconds();
+            o.fock_builds = s.iterations();
+            o.energy_final = s.energy_final_total();
+        }
+        if (D_all) std::copy(s.density().begin(), s.density().end(), D_all + moff[m]);
+        if (F_all) std::copy(s.fock().begin(), s.fock().end(), F_all + moff[m]);
+    }
+    if (grad_all) {
+        const auto tg = clk::now();
+        for (int m = 0; m < nmol; ++m)
+            std::copy(sol[m]->density().begin(), sol[m]->density().end(), D.begin() + moff[m]);
+        check(hfg_set_precision(ctx, opt.precision));
+        int natot = 0;
+        std::vector<int> g0(nmol);
+        for (int m = 0; m < nmol; ++m) {
+            g0[m] = natot;
+            natot += 3 * mols[m]->mol.natom();
+        }
+        std::vector<double> g2(natot);
+        check(hfg_energy_backward(ctx, D.data(), g2.data()));
+#pragma omp parallel for schedule(dynamic)
+        for (int m = 0; m < nmol; ++m) {
+            const Molecule& mol = mols[m]->mol;
+            const int n = nbf[m];
+            const double* Dm = sol[m]->density().data();
+            const double* Fm = sol[m]->fock().data();
+            std::vector<double> DF((size_t)n * n), Q((size_t)n * n);
+            gemm(n, Dm, false, Fm, false, DF.data());
+            gemm(n, DF.data(), false, Dm, false, Q.data());
+            double* g = grad_all + g0[m];
+            std::copy(g2.begin() + g0[m], g2.begin() + g0[m] + 3 * mol.natom(), g);
+            oneint_gradient(mol, Dm, Q.data(), g);
+            nuclear_repulsion_gradient(mol, g);
+        }
+        S.gradient_seconds = secs(tg, clk::now());
+    }
+    if (st) *st = S;
+    int nconv = 0;
+    for (int m = 0; m < nmol; ++m) nconv += conv[m];
+    return nconv == nmol ? 0 : -2;
 }
 
 }  // namespace
@@ -287,6 +461,24 @@ int hfscf_rhf_run(hfscf_mol* m, const hfscf_scf_options* o, const hfscf_callback
         if (energies)
             for (size_t k = 0; k < solver.energies().size(); ++k) energies[k] = solver.energies()[k];
         return conv ? 0 : -2;
+    } catch (const std::exception& e) {
+        return set_err(e.what());
+    }
+}
+
+int hfscf_batch_rhf(int nmol, hfscf_mol* const* mols, const hfscf_scf_options* o, hfscf_scf_result* res,
+                    double* D_all, double* F_all, double* grad_all, hfscf_batch_stats* stats) {
+    if (nmol <= 0 || !mols) return set_err("hfscf_batch_rhf: no molecules");
+    for (int m = 0; m < nmol; ++m)
+        if (!mols[m]) return set_err("hfscf_batch_rhf: null molecule");
+    try {
+        hfscf_scf_options opt;
+        if (o)
+            opt = *o;
+        else
+            hfscf_default_scf_options(&opt);
+        if (opt.backend != HFSCF_BACKEND_GPU) return set_err("hfscf_batch_rhf: batching needs the GPU backend");
+        return batch_rhf(nmol, mols, opt, res, D_all, F_all, grad_all, stats);
     } catch (const std::exception& e) {
         return set_err(e.what());
     }

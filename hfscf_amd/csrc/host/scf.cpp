@@ -1,5 +1,6 @@
 #include "scf.hpp"
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -148,6 +149,10 @@ void hf_solver::update_fock_matrix() {
     auto t1 = std::chrono::steady_clock::now();
     fock_seconds_ = std::chrono::duration<double>(t1 - t0).count();
     fock_total_ += fock_seconds_;
+    after_fock();
+}
+
+void hf_solver::after_fock() {
     diis_->diis_extrapolate(n_, F_.data(), S_.data(), D_.data(), X_.data());
     std::vector<double> tmp((size_t)n_ * n_);
     gemm(n_, X_.data(), true, F_.data(), false, tmp.data());
@@ -155,38 +160,55 @@ void hf_solver::update_fock_matrix() {
     calc_density_matrix();
 }
 
-bool hf_solver::scf_run() {
-    diis_ = std::make_unique<diis_solver>(opt_.diis_range);
-    bool converged = false;
-    for (;;) {
-        update_fock_matrix();
-        const double rms = diis_->current_rms();
-        if (opt_.verbose)
-            std::fprintf(stderr, "  iter %3d  E = %.12f  dE = %.3e  rms = %.3e  fock %.4fs\n", iteration_, e_cur_,
-                         e_cur_ - e_prev_, rms, fock_seconds_);
-        if (rms < opt_.rms_tol && std::fabs(e_cur_ - e_prev_) < opt_.energy_tol) {
-            converged = true;
-            break;
-        }
-        if (iteration_ >= opt_.max_iter) break;
-    }
-    if (opt_.final_build) {
-        jk_->set_final(true);
-        std::vector<double> Ff((size_t)n_ * n_);
-        jk_->fock(H_.data(), D_.data(), 0, Ff.data());
-        double e = 0.0;
-        for (size_t k = 0; k < Ff.size(); ++k) e += D_[k] * (H_[k] + Ff[k]);
-        e_final_ = e;
-        jk_->set_final(false);
-    } else {
-        e_final_ = e_cur_;
-    }
+void hf_solver::begin() { diis_ = std::make_unique<diis_solver>(opt_.diis_range); }
+
+void hf_solver::step(const double* F, double fock_seconds) {
+    ++iteration_;
+    std::copy(F, F + (size_t)n_ * n_, F_.begin());
+    fock_seconds_ = fock_seconds;
+    fock_total_ += fock_seconds;
+    after_fock();
+}
+
+bool hf_solver::converged() const {
+    // hfscf_hf_solver.cpp:415
+    return diis_->current_rms() < opt_.rms_tol && std::fabs(e_cur_ - e_prev_) < opt_.energy_tol;
+}
+
+void hf_solver::finish(double e_final_elec) {
+    e_final_ = e_final_elec;
     // final coefficients from the last orthogonalised Fock matrix (hfscf_hf_solver.cpp:469-473)
     std::vector<double> U((size_t)n_ * n_);
     eps_.resize(n_);
     eigh(n_, Fp_.data(), eps_.data(), U.data());
     gemm(n_, X_.data(), false, U.data(), false, C_.data());
-    return converged;
+}
+
+bool hf_solver::scf_run() {
+    begin();
+    bool conv = false;
+    for (;;) {
+        update_fock_matrix();
+        if (opt_.verbose)
+            std::fprintf(stderr, "  iter %3d  E = %.12f  dE = %.3e  rms = %.3e  fock %.4fs\n", iteration_, e_cur_,
+                         e_cur_ - e_prev_, diis_->current_rms(), fock_seconds_);
+        if (converged()) {
+            conv = true;
+            break;
+        }
+        if (iteration_ >= opt_.max_iter) break;
+    }
+    double e_final = e_cur_;
+    if (opt_.final_build) {
+        jk_->set_final(true);
+        std::vector<double> Ff((size_t)n_ * n_);
+        jk_->fock(H_.data(), D_.data(), 0, Ff.data());
+        e_final = 0.0;
+        for (size_t k = 0; k < Ff.size(); ++k) e_final += D_[k] * (H_[k] + Ff[k]);
+        jk_->set_final(false);
+    }
+    finish(e_final);
+    return conv;
 }
 
 std::vector<double> calc_scf_gradient_rhf(const Molecule& mol, const double* D, const double* F, JKBuilder* jk) {

@@ -60,6 +60,18 @@ class hf_solver {
     void init_data();  // hfscf: src/scfsolver/hfscf_hf_solver.cpp:79-182 (CORE guess)
     bool scf_run();    // hfscf: src/scfsolver/hfscf_hf_solver.cpp:389-476
 
+    // The iteration of scf_run split for drivers that build the Fock matrices of many
+    // molecules at once: begin() starts the loop; step(F) takes the Fock matrix of the current
+    // density and does what update_fock_matrix does after the build (DIIS, new density,
+    // energy); converged() is hfscf's test; finish(e) stores the energy of the final build
+    // (sum D (H + F(D))) and forms the final coefficients.
+    void begin();
+    void step(const double* F, double fock_seconds);
+    bool converged() const;
+    void finish(double e_final_elec);
+    bool wants_incremental() const { return opt_.incremental && iteration_ >= 1; }
+    const ScfOptions& options() const { return opt_; }
+
     double energy_electronic() const { return e_cur_; }
     // E = sum D (H + F(D)) from one more full-precision Fock build at the converged density
     // (stationary in D, so free of the DIIS-extrapolation noise in hfscf's energy expression)
@@ -78,6 +90,7 @@ class hf_solver {
 
    protected:
     virtual void update_fock_matrix();  // hfscf: src/scfsolver/hfscf_hf_solver.cpp:254-286
+    void after_fock();                  // DIIS, F' = X^T F X, density and energy
     void calc_density_matrix();         // hfscf: src/scfsolver/hfscf_hf_solver.cpp:206-242
     void calc_scf_energy();             // hfscf: src/scfsolver/hfscf_hf_solver.cpp:244-252
 

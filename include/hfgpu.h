@@ -51,7 +51,8 @@ typedef struct hfg_options {
                              bound sqrt(ab|ab) sqrt(cd|cd) max(4|dD_ab|, 4|dD_cd|, |dD_ac|, |dD_ad|,
                              |dD_bc|, |dD_bd|) is below this (the bound of hfscf's direct digest,
                              hfscf_hf_solver.cpp:317-324); 0 = off (default) */
-    int reserved[6];
+    int graph;            /* 1 = capture each J/K build configuration once as a HIP graph and replay it */
+    int reserved[5];
 } hfg_options;
 
 /* Fill opt with defaults (FP64, integral_tol 1e-14, screening on, no density screening). */
@@ -106,6 +107,40 @@ int hfg_eri_shell_quartet(hfg_ctx* ctx, int s1, int s2, int s3, int s4, double* 
 
 /* Cartesian Schwarz diagonal (pq|pq), ncart x ncart (hfscf's Erios::screen). */
 int hfg_schwarz(hfg_ctx* ctx, double* screen);
+
+/* ---- Batching mode: many independent molecules in one context ----------------------------
+ * Molecules are added one by one (basis as for hfg_set_basis, offsets within the molecule, and
+ * the molecule's own geometry), then committed. Every shell quartet of every molecule then
+ * goes through one persistent launch per angular-momentum class: work items of all molecules
+ * of that class sit in one device queue. hfg_build_jk, hfg_fock, hfg_energy and
+ * hfg_energy_backward take and return the per-molecule matrices concatenated in molecule order
+ * (n_m x n_m each, offsets from hfg_batch_layout) and gradients concatenated (natom_m x 3).
+ * A single-molecule context is the case of one molecule. No hfscf counterpart: hfscf solves
+ * one molecule per process. */
+
+/* Returns the molecule index (>= 0) or a negative status. natom <= 128 per molecule. */
+int hfg_batch_add_molecule(hfg_ctx* ctx, int nshell, const int* L, const int* atom_of_shell, const int* nprim,
+                           const double* alpha, const double* coef, int pure, const int* cart_offset,
+                           const int* pure_offset, int natom, const double* xyz_bohr);
+/* Pair data, Schwarz screening and work decomposition of every added molecule. */
+int hfg_batch_commit(hfg_ctx* ctx);
+/* Molecule count, and per molecule its basis size, matrix offset and atom count (any may be NULL). */
+int hfg_batch_layout(const hfg_ctx* ctx, int* nmol, int* nbf, long long* moff, int* natom);
+
+/* Autotuner: times every class launch alone over candidate launch shapes (threads per
+ * workgroup x persistent workgroups) on the density of the most recent build and keeps the
+ * fastest per class. what: 1 = J/K, 2 = gradient, 3 = both. */
+int hfg_autotune(hfg_ctx* ctx, int what);
+/* Current launch shape per class (la*1000+lb*100+lc*10+ld, grid, block for J/K and gradient;
+ * grid 0 = one workgroup per work item). Returns the number of classes. */
+int hfg_launch_shapes(const hfg_ctx* ctx, int cap, int* la_lb_lc_ld, int* jk_grid, int* jk_block, int* grad_grid,
+                      int* grad_block);
+
+/* Per class of the last J/K build with direct launches (graph replays are not bracketed): class
+ * la*1000+lb*100+lc*10+ld, surviving quartets, model FLOPs and the class launch's duration
+ * (HIP events on its stream; -1 when not timed). Returns the number of classes. */
+int hfg_class_stats(hfg_ctx* ctx, int cap, int* la_lb_lc_ld, long long* quartets, double* model_flops,
+                    double* kernel_ms);
 
 /* Statistics of the last hfg_build_jk / hfg_fock: surviving shell quartets, model FLOPs
  * (the algorithmic FLOP model over surviving quartets), J/K kernel seconds (HIP events). */

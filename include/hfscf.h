@@ -45,7 +45,9 @@ typedef struct hfscf_scf_options {
     double integral_tol;
     int final_fp64;       /* one more full FP64 Fock build at the converged density (energy_final) */
     double density_tol;   /* density-weighted quartet screening of incremental builds (hfg_options) */
-    int reserved[5];
+    int graph;            /* replay each J/K build configuration from a captured HIP graph */
+    int autotune;         /* batched runs: autotune the class launch shapes after the first build */
+    int reserved[3];
 } hfscf_scf_options;
 
 typedef struct hfscf_scf_result {
@@ -90,6 +92,27 @@ int hfscf_rhf_run(hfscf_mol* m, const hfscf_scf_options* o, const hfscf_callback
 /* RHF gradient (natom x 3) from converged D and F. */
 int hfscf_rhf_gradient(hfscf_mol* m, const hfscf_scf_options* o, const hfscf_callbacks* cb, const double* D,
                        const double* F, double* grad);
+
+/* Batching mode (GPU backend): the SCF of every molecule runs in lockstep with one batched
+ * GPU Fock build per iteration for all molecules (hfgpu batching mode); a molecule that has
+ * converged keeps its density, so its incremental builds have nothing left to add. Each
+ * molecule follows exactly the iteration of hfscf_rhf_run with the same options.
+ * res[nmol] receives per-molecule results. D_all / F_all (concatenated n_m x n_m, may be NULL)
+ * receive densities and last Fock matrices; grad_all (concatenated natom_m x 3, may be NULL)
+ * the RHF gradients from one batched adjoint launch. No hfscf counterpart (hfscf runs one
+ * molecule per process). */
+typedef struct hfscf_batch_stats {
+    double setup_seconds;     /* context, pair data, screening, work decomposition */
+    double scf_seconds;       /* SCF loop including the final builds */
+    double fock_seconds;      /* inside batched Fock builds */
+    double gradient_seconds;  /* batched gradient (two-electron launch + host one-electron terms) */
+    double autotune_seconds;
+    int iterations;           /* batched builds in the SCF loop */
+    int work_items;
+    long long quartets;       /* surviving shell quartets of one full build of all molecules */
+} hfscf_batch_stats;
+int hfscf_batch_rhf(int nmol, hfscf_mol* const* mols, const hfscf_scf_options* o, hfscf_scf_result* res,
+                    double* D_all, double* F_all, double* grad_all, hfscf_batch_stats* stats);
 
 const char* hfscf_last_error(void);
 

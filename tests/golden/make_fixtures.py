@@ -1,6 +1,6 @@
 """Write the oracle's golden vectors used by the GPU parity tests (run on a CPU host).
 
-    python tests/golden/make_fixtures.py [cfg1 cfg2 cfg3 grad]
+    python tests/golden/make_fixtures.py [cfg1 cfg2 cfg3 grad h2o16]
 
 Each fixture is a small .npz of inputs and expected outputs produced by the oracle
 (the CPU restatement of hfscf's two-electron algorithm) through the host RHF loop:
@@ -9,6 +9,7 @@ Each fixture is a small .npz of inputs and expected outputs produced by the orac
                          random density
   cfg3_c12h26_631gss.npz the same for C12H26 / 6-31G**
   grad_h2o.npz           two-electron and total RHF gradients of water (3 bases)
+  grad_h2o16.npz         (H2O)16 / cc-pVDZ converged energy and analytic RHF gradient
 """
 import os
 import sys
@@ -86,6 +87,23 @@ def grad():
     np.savez_compressed(os.path.join(HERE, "grad_h2o.npz"), **out)
 
 
+def h2o16():
+    # (H2O)16 / cc-pVDZ: converged SCF (stored-ERI oracle, about 22 GB of ERIs) and the oracle's
+    # analytic RHF gradient at that density (on-the-fly derivative ERIs)
+    mol = H.Molecule.from_xyz(os.path.join(HERE, "h2o_16.xyz"), "cc-pvdz")
+    orc = oracle.Oracle(mol.shells(), mol.geometry()[1], mol.pure, 1e-14, screen=True)
+    t = time.time()
+    orc.build_eri()
+    print("h2o16 eri build", time.time() - t, flush=True)
+    r = H.rhf(mol, jk=orc.jk, diis_range=8, energy_tol=1e-12, rms_tol=1e-10)
+    print("h2o16 energy", r["energy"], r["energy_final"], r["iterations"], flush=True)
+    t = time.time()
+    g = H.rhf_gradient(mol, r["D"], r["F"], jk=orc.jk, grad2e=orc.grad2e)
+    print("h2o16 gradient", time.time() - t, np.abs(g.sum(0)).max(), flush=True)
+    np.savez_compressed(os.path.join(HERE, "grad_h2o16.npz"), energy=r["energy"], energy_final=r["energy_final"],
+                        iterations=r["iterations"], g=g)
+
+
 if __name__ == "__main__":
     which = sys.argv[1:] or ["cfg1", "cfg2", "grad"]
     if "cfg1" in which:
@@ -96,3 +114,5 @@ if __name__ == "__main__":
         big("cfg2_h2o8_ccpvdz.npz", "h2o_8.xyz", "cc-pvdz")
     if "cfg3" in which:
         big("cfg3_c12h26_631gss.npz", "c12h26.xyz", "6-31g**")
+    if "h2o16" in which:
+        h2o16()

@@ -3,6 +3,7 @@ beyond device discovery)."""
 import os
 import re
 
+import numpy as np
 import pytest
 
 import hfscf_amd as H
@@ -84,3 +85,19 @@ def test_struct_layouts_match_headers(tmp_path):
         assert got[(cname, "size")] == C.sizeof(py), cname
         for fname, _ in py._fields_:
             assert got[(cname, fname)] == getattr(py, fname).offset, (cname, fname)
+
+
+def test_batch_geometry_frames():
+    # the batching workload: 512 seeded (H2O)n clusters, n in 1..10 (seed 5), angstrom frames
+    frames = H.read_xyz_frames(os.path.join(ROOT, "tests", "golden", "batch_h2o.xyz"))
+    assert len(frames) == 512
+    ns = [len(Z) // 3 for Z, _ in frames]
+    assert min(ns) == 1 and max(ns) == 10
+    for Z, X in frames:
+        assert len(Z) % 3 == 0 and X.shape == (len(Z), 3)
+        assert (Z[0::3] == 8).all() and (Z[1::3] == 1).all() and (Z[2::3] == 1).all()
+        # O-H bonds of 0.9572 angstrom in bohr
+        d = np.linalg.norm(X[1::3] - X[0::3], axis=1) / H.ANGSTROM_TO_BOHR
+        assert np.abs(d - 0.9572).max() < 1e-9
+    mol = H.Molecule.from_arrays(*frames[0], "6-31g*")
+    assert mol.nbf == 19 * (len(frames[0][0]) // 3)

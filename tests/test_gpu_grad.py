@@ -47,11 +47,11 @@ def test_grad2e_dimer_live_oracle():
     assert d < 1e-9
 
 
-def _energy(mol, xyz):
+def _energy(mol, xyz, **opts):
     mol.set_geometry(xyz)
-    r = H.rhf(mol, diis_range=8, energy_tol=1e-12, rms_tol=1e-11)
+    r = H.rhf(mol, diis_range=8, energy_tol=1e-12, rms_tol=1e-11, **opts)
     assert r["converged"]
-    return r["energy"]
+    return r["energy_final"]
 
 
 @pytest.mark.parametrize("basis", ["cc-pvdz", "6-31g*"])
@@ -70,6 +70,47 @@ def test_gradient_finite_difference_water(basis):
         Xm[atom, d] -= h
         fd = (_energy(mol, Xp) - _energy(mol, Xm)) / (2 * h)
         print(f"{basis} atom {atom} dir {d}: analytic {g[atom, d]:.10f} fd {fd:.10f} diff {g[atom, d] - fd:.2e}",
+              flush=True)
+        assert abs(g[atom, d] - fd) < 2e-6
+    mol.set_geometry(X0)
+
+
+def test_h2o16_gradient_fd_mixed_and_oracle():
+    # differentiation workload: (H2O)16 / cc-pVDZ (384 functions, 48 atoms). FP64 analytic
+    # gradient against central differences of the GPU energy on 6 seeded coordinates (seed 4,
+    # h = 1e-4 bohr), the mixed-precision gradient against FP64 (bar 1e-5 Eh/bohr), and the
+    # oracle's analytic gradient when its fixture is present (bar 1e-8 Eh/bohr)
+    mol = H.Molecule.from_xyz(golden("h2o_16.xyz"), "cc-pvdz")
+    assert mol.nbf == 384 and mol.natom == 48
+    X0 = mol.geometry()[1].copy()
+    opts = dict(diis_range=8, energy_tol=1e-12, rms_tol=1e-10)
+    r = H.rhf(mol, **opts)
+    assert r["converged"]
+    g = H.rhf_gradient(mol, r["D"], r["F"])
+    assert np.abs(g.sum(0)).max() < 1e-8
+    rm = H.rhf(mol, precision=H.MIXED, incremental=1, **opts)
+    gm = H.rhf_gradient(mol, rm["D"], rm["F"], precision=H.MIXED)
+    print(f"(H2O)16: E {r['energy_final']:.10f}, mixed dE {rm['energy_final'] - r['energy_final']:.2e}, "
+          f"mixed gradient max |dg| {np.abs(gm - g).max():.2e}", flush=True)
+    assert abs(rm["energy_final"] - r["energy_final"]) <= 1e-6
+    assert np.abs(gm - g).max() <= 1e-5
+    import os
+    path = golden("grad_h2o16.npz")
+    if os.path.exists(path):
+        f = np.load(path)
+        print(f"(H2O)16 vs oracle: dE {r['energy_final'] - float(f['energy_final']):.2e}, "
+              f"max |dg| {np.abs(g - f['g']).max():.2e}", flush=True)
+        assert abs(r["energy_final"] - float(f["energy_final"])) < 1e-9
+        assert np.abs(g - f["g"]).max() < 1e-8
+    rng = np.random.default_rng(4)
+    h = 1e-4
+    for c in sorted(rng.choice(mol.natom * 3, size=6, replace=False).tolist()):
+        atom, d = divmod(c, 3)
+        Xp, Xm = X0.copy(), X0.copy()
+        Xp[atom, d] += h
+        Xm[atom, d] -= h
+        fd = (_energy(mol, Xp) - _energy(mol, Xm)) / (2 * h)
+        print(f"(H2O)16 atom {atom} dir {d}: analytic {g[atom, d]:.10f} fd {fd:.10f} diff {g[atom, d] - fd:.2e}",
               flush=True)
         assert abs(g[atom, d] - fd) < 2e-6
     mol.set_geometry(X0)

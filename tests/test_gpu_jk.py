@@ -150,3 +150,19 @@ def test_cfg2_h2o8_ccpvdz_fixture():
           f"{scf['energy'] - float(f['energy']):.2e}) iterations {scf['iterations']}", flush=True)
     assert scf["converged"]
     assert abs(de) < 1e-9
+
+
+def test_cfg3_c12h26_fixture_rows():
+    # C12H26 / 6-31G** (Cartesian d, 310 functions): J/K rows of a seeded density against the
+    # oracle, with hfscf's screening (including the Cartesian diagonal taken from the screen)
+    f = np.load(golden("cfg3_c12h26_631gss.npz"))
+    mol = _mol("c12h26.xyz", "6-31g**")
+    assert mol.nbf == 310
+    D = _seeded_density(mol.nbf, int(f["seed"]))
+    J, K = H.GpuContext(mol).build_jk(D)
+    rows = f["rows"]
+    dj, dk = np.abs(J[rows] - f["J_rows"]).max(), np.abs(K[rows] - f["K_rows"]).max()
+    print(f"cfg3 J/K rows: |dJ| {dj:.2e} |dK| {dk:.2e}; sum D.J {np.sum(D * J) - float(f['EJ']):.2e}, "
+          f"sum D.K {np.sum(D * K) - float(f['EK']):.2e}", flush=True)
+    assert dj < 1e-11 and dk < 1e-11
+    assert abs(np.sum(D * J) - float(f["EJ"])) < 1e-9 and abs(np.sum(D * K) - float(f["EK"])) < 1e-9
