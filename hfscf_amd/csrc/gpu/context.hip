@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -13,6 +14,7 @@
 #include "boys.hpp"
 #include "context.hpp"
 #include "flopmodel.hpp"
+#include "launch_bounds.hpp"
 
 using namespace hfg;
 
@@ -21,6 +23,21 @@ namespace {
 int fail(hfg_ctx* c, int code, const std::string& msg) {
     if (c) c->err = msg;
     return code;
+}
+
+// One stream per device, shared by every context of the process, for the kernels that may
+// spill to scratch (the d-shell class kernels and the Schwarz diagonals). The runtime gives each
+// hardware queue its own scratch from a limited per-device pool and keeps it; spilling kernels
+// spread over several queues (several streams, several contexts) exhausted that pool
+// (HSA_STATUS_ERROR_OUT_OF_RESOURCES). On one stream only one queue holds scratch. It lives
+// until the process ends.
+hipStream_t shared_scratch_stream(int device) {
+    static std::mutex mu;
+    static std::vector<hipStream_t> streams;
+    std::lock_guard<std::mutex> lock(mu);
+    if ((int)streams.size() <= device) streams.resize(device + 1, nullptr);
+    if (!streams[device] && hipStreamCreate(&streams[device]) != hipSuccess) streams[device] = nullptr;
+    return streams[device];
 }
 
 int ket_count_host(const double* s, int n, double si, double tol) {
@@ -217,11 +234,11 @@ int setup_geometry(hfg_ctx* c) {
             a.pp2 = c->d_pp2.p;
             a.boys = c->d_boys.p;
             a.out = ddiag.p;
-            HFG_CHECK(c, launch_screen(cls_range[r].first / 3, cls_range[r].first % 3, a, c->stream));
+            HFG_CHECK(c, launch_screen(cls_range[r].first / 3, cls_range[r].first % 3, a, c->heavy));
         }
         HFG_CHECK(c, hipMemcpyAsync(diag.data(), ddiag.p, diag.size() * sizeof(double), hipMemcpyDeviceToHost,
-                                    c->stream));
-        HFG_CHECK(c, hipStreamSynchronize(c->stream));
+                                    c->heavy));
+        HFG_CHECK(c, hipStreamSynchronize(c->heavy));
         dsp.release();
         doff.release();
         ddiag.release();
@@ -472,14 +489,25 @@ int setup_geometry(hfg_ctx* c) {
         std::vector<ClassWork> sorted;
         for (size_t k : ord) sorted.push_back(c->classes[k]);
         c->classes.swap(sorted);
-        double load[kSideStreams] = {0};
-        c->class_stream.assign(c->classes.size(), 0);
-        for (size_t k = 0; k < c->classes.size(); ++k) {
-            int best = 0;
-            for (int s = 1; s < kSideStreams; ++s)
-                if (load[s] < load[best]) best = s;
-            c->class_stream[k] = best;
-            load[best] += c->classes[k].flops;
+        // classes that may spill to scratch go to the device's shared stream (-1); the others
+        // are balanced over the context's side streams (separately for the J/K and the gradient
+        // launches, whose spills differ)
+        for (int grad = 0; grad < 2; ++grad) {
+            double load[kSideStreams] = {0};
+            std::vector<int>& map = grad ? c->grad_stream : c->class_stream;
+            map.assign(c->classes.size(), 0);
+            for (size_t k = 0; k < c->classes.size(); ++k) {
+                const ClassWork& cw = c->classes[k];
+                if (class_spills(cw.la, cw.lb, cw.lc, cw.ld, grad != 0)) {
+                    map[k] = -1;
+                    continue;
+                }
+                int best = 0;
+                for (int s = 1; s < kSideStreams; ++s)
+                    if (load[s] < load[best]) best = s;
+                map[k] = best;
+                load[best] += grad ? cw.grad_flops : cw.flops;
+            }
         }
     }
     // gather lists (one output tile per block pair) and per-molecule item lists
@@ -827,7 +855,9 @@ int hfg_create(hfg_ctx** out, int device, const hfg_options* opt) {
     }
     if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess ||
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
-        hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_heavy_join, hipEventDisableTiming) != hipSuccess ||
+        (c->heavy = shared_scratch_stream(device)) == nullptr) {
         hfg_destroy(c);
         return HFG_ERR_HIP;
     }
@@ -1165,6 +1195,8 @@ void hfg_destroy(hfg_ctx* c) {
         if (c->ev_join[k]) (void)hipEventDestroy(c->ev_join[k]);
         if (c->side[k]) (void)hipStreamDestroy(c->side[k]);
     }
+    if (c->heavy) (void)hipStreamSynchronize(c->heavy);
+    if (c->ev_heavy_join) (void)hipEventDestroy(c->ev_heavy_join);
     if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);

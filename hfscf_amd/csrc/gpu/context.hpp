@@ -87,7 +87,11 @@ struct hfg_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     hipStream_t side[hfg::kSideStreams] = {};
     hipEvent_t ev_fork = nullptr, ev_join[hfg::kSideStreams] = {};
-    std::vector<int> class_stream;  // side stream of each class launch
+    // the device's shared stream for kernels that may spill to scratch (see shared_scratch_stream)
+    hipStream_t heavy = nullptr;
+    hipEvent_t ev_heavy_join = nullptr;
+    std::vector<int> class_stream;  // side stream of each class's J/K launch (-1: the shared stream)
+    std::vector<int> grad_stream;   // the same for the gradient launches
     hfg_options opt{};
     std::string err;
     bool have_basis = false, have_geom = false;
@@ -167,18 +171,27 @@ namespace hfg {
 // to `main`): classes are independent until the gather. Under stream capture the fork/join
 // events make the side-stream launches part of the captured graph.
 template <typename F>
-inline hipError_t for_each_class_concurrent(hfg_ctx* c, hipStream_t main, F&& fn) {
+inline hipError_t for_each_class_concurrent(hfg_ctx* c, hipStream_t main, F&& fn, bool grad = false) {
+    const std::vector<int>& streams = grad ? c->grad_stream : c->class_stream;
     hipError_t e = hipEventRecord(c->ev_fork, main);
     if (e != hipSuccess) return e;
     for (int k = 0; k < kSideStreams; ++k)
         if ((e = hipStreamWaitEvent(c->side[k], c->ev_fork, 0)) != hipSuccess) return e;
+    if ((e = hipStreamWaitEvent(c->heavy, c->ev_fork, 0)) != hipSuccess) return e;
     for (size_t i = 0; i < c->classes.size(); ++i)
-        if ((e = fn(c->classes[i], c->side[c->class_stream[i]])) != hipSuccess) return e;
+        if ((e = fn(c->classes[i], streams[i] < 0 ? c->heavy : c->side[streams[i]])) != hipSuccess) return e;
     for (int k = 0; k < kSideStreams; ++k) {
         if ((e = hipEventRecord(c->ev_join[k], c->side[k])) != hipSuccess) return e;
         if ((e = hipStreamWaitEvent(main, c->ev_join[k], 0)) != hipSuccess) return e;
     }
-    return hipSuccess;
+    if ((e = hipEventRecord(c->ev_heavy_join, c->heavy)) != hipSuccess) return e;
+    return hipStreamWaitEvent(main, c->ev_heavy_join, 0);
+}
+
+// Stream of a class launch (J/K or gradient).
+inline hipStream_t class_launch_stream(hfg_ctx* c, size_t k, bool grad) {
+    const int s = (grad ? c->grad_stream : c->class_stream)[k];
+    return s < 0 ? c->heavy : c->side[s];
 }
 
 // Run class k alone (J/K, or the gradient when grad) with launch shape sh on the main stream,

@@ -75,7 +75,7 @@ int run_gradient(hfg_ctx* c, const double* D, double* grad) {
         b.items = c->d_items.p + cw.dev_off;
         b.queue = c->d_queue.p + ncls + (k++);
         return launch_grad(cw.la, cw.lb, cw.lc, cw.ld, c->pure, mixed, b, (int)cw.items.size(), cw.grad_shape, st);
-    }));
+    }, true));
     HFG_CHECK(c, hipEventRecord(c->ev1, c->stream));
     GradReduceArgs r{};
     r.slab = c->d_grad_slab.p;
@@ -100,23 +100,25 @@ int run_one_class(hfg_ctx* c, size_t k, bool grad, LaunchShape sh) {
     const ClassWork& cw = c->classes[k];
     const int mixed = c->opt.precision == HFG_MIXED ? 1 : 0;
     int* q = c->d_queue.p + (grad ? c->classes.size() : 0) + k;
-    HFG_CHECK(c, launch_zero_ints(q, 1, c->stream));
-    HFG_CHECK(c, hipEventRecord(c->ev0, c->stream));
+    hipStream_t st = class_launch_stream(c, k, grad);
+    HFG_CHECK(c, hipStreamSynchronize(c->stream));
+    HFG_CHECK(c, launch_zero_ints(q, 1, st));
+    HFG_CHECK(c, hipEventRecord(c->ev0, st));
     if (grad) {
         HFG_CHECK(c, c->d_grad_slab.ensure((size_t)std::max<long long>(c->grad_slab_doubles, 1)));
         GradArgs b = grad_args(c);
         b.slab = c->d_grad_slab.p;
         b.items = c->d_items.p + cw.dev_off;
         b.queue = q;
-        HFG_CHECK(c, launch_grad(cw.la, cw.lb, cw.lc, cw.ld, c->pure, mixed, b, (int)cw.items.size(), sh, c->stream));
+        HFG_CHECK(c, launch_grad(cw.la, cw.lb, cw.lc, cw.ld, c->pure, mixed, b, (int)cw.items.size(), sh, st));
     } else {
         JkArgs b = jk_args(c);
         b.items = c->d_items.p + cw.dev_off;
         b.queue = q;
-        HFG_CHECK(c, launch_jk(cw.la, cw.lb, cw.lc, cw.ld, c->pure, mixed, b, (int)cw.items.size(), sh, c->stream));
+        HFG_CHECK(c, launch_jk(cw.la, cw.lb, cw.lc, cw.ld, c->pure, mixed, b, (int)cw.items.size(), sh, st));
     }
-    HFG_CHECK(c, hipEventRecord(c->ev1, c->stream));
-    HFG_CHECK(c, hipStreamSynchronize(c->stream));
+    HFG_CHECK(c, hipEventRecord(c->ev1, st));
+    HFG_CHECK(c, hipStreamSynchronize(st));
     return HFG_OK;
 }
 

@@ -1,5 +1,6 @@
 // Run-time dispatch of the adjoint gradient kernels and the fixed-order slab reduction.
 #include "grad_impl.hpp"
+#include "launch_bounds.hpp"
 
 namespace hfg {
 
@@ -14,8 +15,9 @@ HFG_FOR_CLASSES(HFG_DECL)
 #undef HFG_DECL1
 
 hipError_t launch_grad(int la, int lb, int lc, int ld, bool pure, int mixed, const GradArgs& a, int nitems,
-                       LaunchShape sh, hipStream_t s) {
+                       LaunchShape want, hipStream_t s) {
     if (nitems == 0) return hipSuccess;
+    const LaunchShape sh = scratch_safe_shape(la, lb, lc, ld, true, want, nitems);
 #define HFG_CASE(A, B, C, D)                                                                        \
     if (la == A && lb == B && lc == C && ld == D) {                                                 \
         if (pure)                                                                                   \

@@ -1,5 +1,6 @@
 // Run-time dispatch from (la lb|lc ld), basis type and precision to the specialised J/K kernel.
 #include "jk_impl.hpp"
+#include "launch_bounds.hpp"
 
 namespace hfg {
 
@@ -14,8 +15,9 @@ HFG_FOR_CLASSES(HFG_DECL)
 #undef HFG_DECL1
 
 hipError_t launch_jk(int la, int lb, int lc, int ld, bool pure, int mixed, const JkArgs& a, int nitems,
-                     LaunchShape sh, hipStream_t s) {
+                     LaunchShape want, hipStream_t s) {
     if (nitems == 0) return hipSuccess;
+    const LaunchShape sh = scratch_safe_shape(la, lb, lc, ld, false, want, nitems);
 #define HFG_CASE(A, B, C, D)                                                                      \
     if (la == A && lb == B && lc == C && ld == D) {                                               \
         if (pure)                                                                                 \

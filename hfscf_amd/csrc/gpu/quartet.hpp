@@ -6,9 +6,48 @@
 
 namespace hfg {
 
+// Boys values scaled by `scale` as boys_eval, with the large-T branch on one reciprocal square
+// root (F_0 = sqrt(pi/4) T^-1/2, 1/T = (T^-1/2)^2) instead of a division and a square root.
+template <int M, typename R>
+__device__ __forceinline__ void boys_eval_rs(double T, const double* __restrict__ tab, double scale, R* F) {
+    static_assert(M <= kBoysMaxM, "Boys order beyond table");
+    double f[M + 1];
+    if (T < kBoysTmax) {
+        const int k = __double2int_rn(T * kBoysStepInv);
+        const double dt = (double)k * (1.0 / kBoysStepInv) - T;
+        const double* row = tab + k * kBoysCols + M;
+        double s = row[6];
+        s = row[5] + s * dt * (1.0 / 6.0);
+        s = row[4] + s * dt * (1.0 / 5.0);
+        s = row[3] + s * dt * (1.0 / 4.0);
+        s = row[2] + s * dt * (1.0 / 3.0);
+        s = row[1] + s * dt * (1.0 / 2.0);
+        s = row[0] + s * dt;
+        f[M] = s;
+        if constexpr (M > 0) {
+            const double eT = exp(-T);
+#pragma unroll
+            for (int m = M - 1; m >= 0; --m) f[m] = (2.0 * T * f[m + 1] + eT) * (1.0 / (2 * m + 1));
+        }
+    } else {
+        const double r = rsqrt(T);
+        f[0] = 0.88622692545275801365 * r;  // sqrt(pi)/2
+        if constexpr (M > 0) {
+            const double eT = exp(-T);
+            const double h = 0.5 * r * r;
+#pragma unroll
+            for (int m = 0; m < M; ++m) f[m + 1] = ((2 * m + 1) * f[m] - eT) * h;
+        }
+    }
+#pragma unroll
+    for (int m = 0; m <= M; ++m) F[m] = (R)(scale * f[m]);
+}
+
 // Contracted Cartesian block (ab|cd) of one shell quartet, out[a][b][c][d] (ncart each).
 // prim_cut mirrors hfscf's primitive cutoff |pfac| < integral_tol/1e4
-// (hfscf: src/integrals/hfscf_oseri.cpp:73,98-101).
+// (hfscf: src/integrals/hfscf_oseri.cpp:73,98-101): a cut primitive quartet contributes zero.
+// The next ket primitive's data is loaded before the current one is evaluated, and the cut is
+// a multiplication, not a branch, so the loads of one iteration are not split around it.
 template <int LA, int LB, int LC, int LD, typename R>
 HFG_INLINE void eri_cart(const PairDev& bp, const PairDev& kp, const double4* __restrict__ pp0,
                          const double4* __restrict__ pp1, const double4* __restrict__ pp2,
@@ -17,20 +56,24 @@ HFG_INLINE void eri_cart(const PairDev& bp, const PairDev& kp, const double4* __
     using H = Hrr<LA, LB, LC, LD, R>;
     R acc[H::NE][H::NF];
     sfor<H::NE>([&](auto e) { sfor<H::NF>([&](auto f) { acc[e][f] = (R)0; }); });
+    const int kn = kp.np, ks = kp.pstride;
     for (int i = 0; i < bp.np; ++i) {
         const int ia = bp.off + i * bp.pstride;
         const double4 a0 = pp0[ia];
         const double4 a1 = pp1[ia];
         const double o2p = pp2[ia].x;
         const double p = a0.x;
-        for (int j = 0; j < kp.np; ++j) {
-            const int ib = kp.off + j * kp.pstride;
-            const double4 b0 = pp0[ib];
-            const double4 b1 = pp1[ib];
+        int ib = kp.off;
+        double4 b0 = pp0[ib], b1 = pp1[ib];
+        double o2q = (LT > 0) ? pp2[ib].x : 0.0;
+        for (int j = 0; j < kn; ++j) {
+            const int ibn = (j + 1 < kn) ? ib + ks : ib;  // stays in bounds on the last primitive
+            const double4 nb0 = pp0[ibn], nb1 = pp1[ibn];
+            const double no2q = (LT > 0) ? pp2[ibn].x : 0.0;
             const double q = b0.x;
             const double rs = rsqrt(p + q);
-            const double pfac = a1.w * b1.w * rs;
-            if (fabs(pfac) < prim_cut) continue;
+            double pfac = a1.w * b1.w * rs;
+            pfac = (fabs(pfac) < prim_cut) ? 0.0 : pfac;
             const double PQx = a0.y - b0.y, PQy = a0.z - b0.z, PQz = a0.w - b0.w;
             const double r2 = PQx * PQx + PQy * PQy + PQz * PQz;
             const double ipq = rs * rs;
@@ -40,21 +83,25 @@ HFG_INLINE void eri_cart(const PairDev& bp, const PairDev& kp, const double4* __
             if constexpr (LT == 0) {
                 // hfscf floors T at 1e-8 inside F_nu and uses F_n = 1/(2n+1) when r_PQ^2 <= 1e-18
                 T = (r2 > 1e-18) ? fmax(T, 1e-8) : 0.0;
-                boys_eval<0, R>(T, boys_tab, pfac, F);
+                boys_eval_rs<0, R>(T, boys_tab, pfac, F);
                 acc[0][0] += F[0];
             } else {
-                boys_eval<LT, R>(T, boys_tab, pfac, F);
+                boys_eval_rs<LT, R>(T, boys_tab, pfac, F);
                 const double rp = q * ipq, rq = p * ipq;
                 const R PA[3] = {(R)a1.x, (R)a1.y, (R)a1.z};
                 const R QC[3] = {(R)b1.x, (R)b1.y, (R)b1.z};
                 const R WP[3] = {(R)(-rp * PQx), (R)(-rp * PQy), (R)(-rp * PQz)};
                 const R WQ[3] = {(R)(rq * PQx), (R)(rq * PQy), (R)(rq * PQz)};
                 Vrr<LA + LB, LC + LD, R> V;
-                V.run(F, PA, WP, QC, WQ, (R)o2p, (R)rp, (R)pp2[ib].x, (R)rq, (R)(0.5 * ipq));
+                V.run(F, PA, WP, QC, WQ, (R)o2p, (R)rp, (R)o2q, (R)rq, (R)(0.5 * ipq));
                 sfor<H::NE>([&](auto e) {
                     sfor<H::NF>([&](auto f) { acc[e][f] += V.v[H::NE0 + e][H::NF0 + f][0]; });
                 });
             }
+            b0 = nb0;
+            b1 = nb1;
+            o2q = no2q;
+            ib = ibn;
         }
     }
     const R AB[3] = {(R)bp.AB[0], (R)bp.AB[1], (R)bp.AB[2]};
