@@ -224,6 +224,9 @@ int setup_geometry(hfg_ctx* c) {
         HFG_CHECK(c, dsp.upload(sp, c->stream));
         HFG_CHECK(c, doff.upload(off, c->stream));
         HFG_CHECK(c, ddiag.ensure(diag.size()));
+        // the screen kernels run on the shared scratch stream: the primitive data, pair records
+        // and output offsets uploaded on the context's stream must have landed first
+        HFG_CHECK(c, hipStreamSynchronize(c->stream));
         for (size_t r = 0; r + 1 < cls_range.size(); ++r) {
             ScreenArgs a{};
             a.pairs = dsp.p + cls_range[r].second;
