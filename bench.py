@@ -127,9 +127,11 @@ def dominant_roofline(classes):
             "model_gflop_per_launch": c["model_flops"] / 1e9, "kernel_ms": c["kernel_ms"]}
 
 
-def gradient_figures(mol, D, steps, warmup):
+def gradient_figures(mol, D, steps, warmup, autotune=True):
     """FP64 two-electron gradients/s (adjoint kernel) at the workload's density."""
     ctx = H.GpuContext(mol)
+    if autotune:
+        ctx.autotune(2)
     el, kern, g = time_grads(ctx, D, steps, warmup)
     st = ctx.stats()
     ctx.close()
@@ -227,6 +229,8 @@ def main():
     ap.add_argument("--workload", default="h2o8", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default launch shapes and FLOP-model stream balance")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-scf", action="store_true",
                     help="time the builds on a seeded random density instead of the converged one "
@@ -264,6 +268,8 @@ def main():
         log(f"SCF {args.workload}: E = {scf['energy']:.10f} Eh (final build {scf['energy_final']:.10f}), "
             f"{scf['iterations']} iterations, {scf['seconds']:.2f} s")
     ctx = H.GpuContext(mol)
+    if not args.no_autotune:
+        ctx.autotune(1)  # launch shapes and stream balance from measured class times (untimed set-up)
     el, kern = time_builds(ctx, Hc, D, args.steps, args.warmup)
     st = ctx.stats()
     classes = ctx.class_stats()  # the last build's class launches, bracketed by HIP events
@@ -288,7 +294,8 @@ def main():
         "config": {"workload": desc, "nbf": mol.nbf, "shell_quartets": st["quartets"], "precision": "fp64",
                    "scf_energy": scf["energy"], "scf_energy_final": scf["energy_final"],
                    "scf_iterations": scf["iterations"], "scf_seconds": scf["seconds"],
-                   "quartets_per_s": st["quartets"] * builds_s},
+                   "quartets_per_s": st["quartets"] * builds_s,
+                   "launch_shapes": "default" if args.no_autotune else "autotuned"},
         "roofline": dominant_roofline(classes),
         "build_roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP64, "unit": "TFLOP/s",
                            "frac": achieved / PEAK_FP64, "model_gflop_per_build": st["model_flops"] / 1e9,
@@ -306,7 +313,7 @@ def main():
         if "energy_final" in f:
             out["config"]["dE_final_vs_oracle"] = scf["energy_final"] - float(f["energy_final"])
     if not args.no_extra:
-        out["extra"] = gradient_figures(mol, D, max(2, args.steps // 2), 1)
+        out["extra"] = gradient_figures(mol, D, max(2, args.steps // 2), 1, not args.no_autotune)
         if not args.no_mixed:
             out["extra"].update(mixed_figures(args.mixed_workload, max(2, args.steps // 2), 1))
         if args.batch_molecules > 0:

@@ -40,6 +40,31 @@ hipStream_t shared_scratch_stream(int device) {
     return streams[device];
 }
 
+// Streams of the class launches (J/K, or the gradient when grad): the classes that spill run on
+// the device's shared stream (-1); the others are balanced over the side streams, heaviest
+// first onto the least loaded one, by the weights w (the FLOP model at set-up, the measured
+// kernel times after autotuning).
+void assign_streams(hfg_ctx* c, bool grad, const std::vector<double>& w) {
+    std::vector<int>& map = grad ? c->grad_stream : c->class_stream;
+    map.assign(c->classes.size(), 0);
+    std::vector<size_t> ord(c->classes.size());
+    std::iota(ord.begin(), ord.end(), 0);
+    std::stable_sort(ord.begin(), ord.end(), [&](size_t x, size_t y) { return w[x] > w[y]; });
+    double load[kSideStreams] = {0};
+    for (size_t k : ord) {
+        const ClassWork& cw = c->classes[k];
+        if (class_spills(cw.la, cw.lb, cw.lc, cw.ld, grad)) {
+            map[k] = -1;
+            continue;
+        }
+        int best = 0;
+        for (int s = 1; s < kSideStreams; ++s)
+            if (load[s] < load[best]) best = s;
+        map[k] = best;
+        load[best] += w[k];
+    }
+}
+
 int ket_count_host(const double* s, int n, double si, double tol) {
     int lo = 0, hi = n;
     while (lo < hi) {
@@ -484,7 +509,7 @@ int setup_geometry(hfg_ctx* c) {
     }
     c->classes.erase(std::remove_if(c->classes.begin(), c->classes.end(), [](const ClassWork& w) { return w.items.empty(); }),
                      c->classes.end());
-    // classes to side streams, heaviest first onto the least loaded stream
+    // classes heaviest first (the launch order), then onto the streams by the FLOP model
     {
         std::vector<size_t> ord(c->classes.size());
         std::iota(ord.begin(), ord.end(), 0);
@@ -492,26 +517,13 @@ int setup_geometry(hfg_ctx* c) {
         std::vector<ClassWork> sorted;
         for (size_t k : ord) sorted.push_back(c->classes[k]);
         c->classes.swap(sorted);
-        // classes that may spill to scratch go to the device's shared stream (-1); the others
-        // are balanced over the context's side streams (separately for the J/K and the gradient
-        // launches, whose spills differ)
-        for (int grad = 0; grad < 2; ++grad) {
-            double load[kSideStreams] = {0};
-            std::vector<int>& map = grad ? c->grad_stream : c->class_stream;
-            map.assign(c->classes.size(), 0);
-            for (size_t k = 0; k < c->classes.size(); ++k) {
-                const ClassWork& cw = c->classes[k];
-                if (class_spills(cw.la, cw.lb, cw.lc, cw.ld, grad != 0)) {
-                    map[k] = -1;
-                    continue;
-                }
-                int best = 0;
-                for (int s = 1; s < kSideStreams; ++s)
-                    if (load[s] < load[best]) best = s;
-                map[k] = best;
-                load[best] += grad ? cw.grad_flops : cw.flops;
-            }
+        std::vector<double> wj, wg;
+        for (const auto& cw : c->classes) {
+            wj.push_back(cw.flops);
+            wg.push_back(cw.grad_flops);
         }
+        assign_streams(c, false, wj);
+        assign_streams(c, true, wg);
     }
     // gather lists (one output tile per block pair) and per-molecule item lists
     std::vector<std::vector<GatherEntry>> lj(nbp), lk(nbp);
@@ -1020,6 +1032,7 @@ int hfg_autotune(hfg_ctx* c, int what) {
     destroy_graphs(c);
     for (int kind = 1; kind <= 2; ++kind) {
         if (!(what & kind)) continue;
+        std::vector<double> measured(c->classes.size(), 0.0);
         for (size_t k = 0; k < c->classes.size(); ++k) {
             ClassWork& cw = c->classes[k];
             LaunchShape best = kind == 1 ? cw.jk_shape : cw.grad_shape;
@@ -1040,7 +1053,10 @@ int hfg_autotune(hfg_ctx* c, int what) {
                     }
                 }
             (kind == 1 ? cw.jk_shape : cw.grad_shape) = best;
+            measured[k] = best_ms;
         }
+        // rebalance the side streams on the measured times of the chosen shapes
+        assign_streams(c, kind == 2, measured);
     }
     return HFG_OK;
 }
@@ -1171,6 +1187,7 @@ void hfg_destroy(hfg_ctx* c) {
     c->d_dmax.release();
     c->d_grad_slab.release();
     c->d_grad.release();
+    c->d_gws.release();
     c->d_pp0.release();
     c->d_pp1.release();
     c->d_pp2.release();

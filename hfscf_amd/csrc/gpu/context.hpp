@@ -130,6 +130,9 @@ struct hfg_ctx {
     double total_flops = 0, total_grad_flops = 0;
     // device
     DevBuf<double> d_psq, d_boys, d_D, d_J, d_K, d_F, d_H, d_G, d_slab, d_psmean, d_grad_slab, d_grad;
+    // workspace of the gradient classes whose recursion tables live in global memory (their
+    // launches share it and run in order on the shared stream); allocated at the first gradient
+    DevBuf<double> d_gws;
     DevBuf<float> d_dmax;  // largest |density| per shell block (density-weighted screening)
     std::vector<float> dmax;
     DevBuf<double4> d_pp0, d_pp1, d_pp2;

@@ -40,6 +40,26 @@ hipError_t launch_screen(int la, int lb, const ScreenArgs& a, hipStream_t s) {
 }
 
 // ------------------------------------------------------------------ gather
+// Sum of element (u, v) over the work-item tiles [g0, g1) of a gather list, in a fixed order:
+// four interleaved partial sums (entries g0 + 4i + q), so four independent loads are in flight.
+__device__ __forceinline__ double gather_sum(const GatherEntry* __restrict__ list, int g0, int g1,
+                                             const double* __restrict__ slab, int u, int v) {
+    double s[4] = {0.0, 0.0, 0.0, 0.0};
+    int g = g0;
+    for (; g + 3 < g1; g += 4) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const GatherEntry en = list[g + q];
+            s[q] += en.trans ? slab[en.off + (long long)v * en.cols + u] : slab[en.off + (long long)u * en.cols + v];
+        }
+    }
+    for (; g < g1; ++g) {
+        const GatherEntry en = list[g];
+        s[0] += en.trans ? slab[en.off + (long long)v * en.cols + u] : slab[en.off + (long long)u * en.cols + v];
+    }
+    return (s[0] + s[1]) + (s[2] + s[3]);
+}
+
 // One workgroup per output tile (X >= Y): sums every work-item tile that targets it
 // (transposed where the work item held it as (Y, X)) in a fixed order, then writes the
 // symmetric J and K elements.
@@ -56,15 +76,8 @@ __global__ __launch_bounds__(256) void gather_kernel(GatherArgs a) {
     for (int e = threadIdx.x; e < fX * fY; e += blockDim.x) {
         const int u = e / fY, v = e - u * fY;
         if (xy.x == xy.y && u < v) continue;  // diagonal tile: one thread per symmetric element pair
-        double sj = 0.0, sk = 0.0;
-        for (int g = j0; g < j1; ++g) {
-            const GatherEntry en = a.gJ[g];
-            sj += en.trans ? a.slab[en.off + (long long)v * en.cols + u] : a.slab[en.off + (long long)u * en.cols + v];
-        }
-        for (int g = k0; g < k1; ++g) {
-            const GatherEntry en = a.gK[g];
-            sk += en.trans ? a.slab[en.off + (long long)v * en.cols + u] : a.slab[en.off + (long long)u * en.cols + v];
-        }
+        const double sj = gather_sum(a.gJ, j0, j1, a.slab, u, v);
+        const double sk = gather_sum(a.gK, k0, k1, a.slab, u, v);
         const size_t i1 = (size_t)aoX[u] * md.n + aoY[v];
         const size_t i2 = (size_t)aoY[v] * md.n + aoX[u];
         J[i1] = sj;
