@@ -97,6 +97,9 @@ def h2o16():
     print("h2o16 eri build", time.time() - t, flush=True)
     r = H.rhf(mol, jk=orc.jk, diis_range=8, energy_tol=1e-12, rms_tol=1e-10)
     print("h2o16 energy", r["energy"], r["energy_final"], r["iterations"], flush=True)
+    # the energies first (the analytic gradient below takes hours of CPU)
+    np.savez_compressed(os.path.join(HERE, "energy_h2o16.npz"), energy=r["energy"], energy_final=r["energy_final"],
+                        iterations=r["iterations"])
     t = time.time()
     g = H.rhf_gradient(mol, r["D"], r["F"], jk=orc.jk, grad2e=orc.grad2e)
     print("h2o16 gradient", time.time() - t, np.abs(g.sum(0)).max(), flush=True)

@@ -78,8 +78,9 @@ def test_gradient_finite_difference_water(basis):
 def test_h2o16_gradient_fd_mixed_and_oracle():
     # differentiation workload: (H2O)16 / cc-pVDZ (384 functions, 48 atoms). FP64 analytic
     # gradient against central differences of the GPU energy on 6 seeded coordinates (seed 4,
-    # h = 1e-4 bohr), the mixed-precision gradient against FP64 (bar 1e-5 Eh/bohr), and the
-    # oracle's analytic gradient when its fixture is present (bar 1e-8 Eh/bohr)
+    # h = 1e-4 bohr), the mixed-precision gradient against FP64 (bar 1e-5 Eh/bohr), the oracle's
+    # SCF energy (bar 1e-9 Eh) and the oracle's analytic gradient when its fixture is present
+    # (bar 1e-8 Eh/bohr)
     mol = H.Molecule.from_xyz(golden("h2o_16.xyz"), "cc-pvdz")
     assert mol.nbf == 384 and mol.natom == 48
     X0 = mol.geometry()[1].copy()
@@ -95,6 +96,10 @@ def test_h2o16_gradient_fd_mixed_and_oracle():
     assert abs(rm["energy_final"] - r["energy_final"]) <= 1e-6
     assert np.abs(gm - g).max() <= 1e-5
     import os
+    fe = np.load(golden("energy_h2o16.npz"))  # the oracle's SCF (stored ERIs) on this cluster
+    print(f"(H2O)16 vs oracle: dE {r['energy_final'] - float(fe['energy_final']):.2e} "
+          f"({int(fe['iterations'])} oracle iterations)", flush=True)
+    assert abs(r["energy_final"] - float(fe["energy_final"])) < 1e-9
     path = golden("grad_h2o16.npz")
     if os.path.exists(path):
         f = np.load(path)
