@@ -119,7 +119,7 @@ def cpu_baseline(mol, budget_rows=32):
 def dominant_roofline(classes):
     """Roofline of the dominant J/K kernel (the class launch with the longest duration in the last
     build): model FLOPs of its surviving quartets / its HIP-event duration, against FP64 vector peak.
-    HBM traffic per launch comes from the profiler's counters (profiles/), not from this run."""
+    Traffic is None: the profiler's HBM byte counters could not be collected for these kernels."""
     c = max(classes, key=lambda c: c["kernel_ms"])
     ach = c["model_flops"] / (c["kernel_ms"] * 1e-3) / 1e12
     return {"bound": "valu", "kernel": f"jk_kernel ({c['cls'][0]}{c['cls'][1]}|{c['cls'][2]}{c['cls'][3]}) fp64",
@@ -136,8 +136,8 @@ def gradient_figures(mol, D, steps, warmup, autotune=True):
     st = ctx.stats()
     ctx.close()
     log(f"FP64 gradient: {steps / el:.2f}/s, kernel {1e3 * kern / steps:.2f} ms")
-    return {"fp64_gradients_per_s": steps / el, "fp64_gradient_kernel_ms": 1e3 * kern / steps,
-            "fp64_gradient_model_tflops": st["model_flops"] / (kern / steps) / 1e12}
+    return {"main_fp64_gradients_per_s": steps / el, "main_fp64_gradient_kernel_ms": 1e3 * kern / steps,
+            "main_fp64_gradient_model_tflops": st["model_flops"] / (kern / steps) / 1e12}
 
 
 def mixed_figures(workload, steps, warmup):
@@ -154,12 +154,16 @@ def mixed_figures(workload, steps, warmup):
     scf64 = H.rhf(mol, **SCF_OPTS)
     out["scf_fp64_seconds"] = time.perf_counter() - t0
     t0 = time.perf_counter()
-    scfm = H.rhf(mol, precision=H.MIXED, incremental=1, **SCF_OPTS)
+    # FP32 density-difference builds leave about 1e-8 of error in the accumulated Fock matrix, so
+    # the mixed SCF stops at rms 1e-7 (its energy is the final FP64 build's)
+    scfm = H.rhf(mol, precision=H.MIXED, incremental=1, **dict(SCF_OPTS, energy_tol=1e-9, rms_tol=1e-7))
     out["scf_mixed_seconds"] = time.perf_counter() - t0
     out["scf_fp64_fock_seconds"] = scf64["fock_seconds"]
     out["scf_mixed_fock_seconds"] = scfm["fock_seconds"]
     out["scf_fp64_iterations"] = scf64["iterations"]
     out["scf_mixed_iterations"] = scfm["iterations"]
+    out["scf_fp64_converged"] = bool(scf64["converged"])
+    out["scf_mixed_converged"] = bool(scfm["converged"])
     out["scf_mixed_dE_vs_fp64"] = scfm["energy_final"] - scf64["energy_final"]
     out["scf_fock_speedup_mixed_vs_fp64"] = scf64["fock_seconds"] / max(scfm["fock_seconds"], 1e-12)
     log(f"{workload} SCF FP64 {out['scf_fp64_seconds']:.2f} s (Fock {scf64['fock_seconds']:.2f} s), mixed "

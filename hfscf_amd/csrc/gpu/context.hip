@@ -445,7 +445,8 @@ int setup_geometry(hfg_ctx* c) {
                 const int tile = c->blk_nf[A] * c->blk_nf[B] + c->blk_nf[C] * c->blk_nf[D] +
                                  c->blk_nf[A] * c->blk_nf[C] + c->blk_nf[A] * c->blk_nf[D] +
                                  c->blk_nf[B] * c->blk_nf[C] + c->blk_nf[B] * c->blk_nf[D];
-                // per-class item target: a class launch should have >= 256 items when it has the work
+                // per-class item target: a class launch should have >= 256 items when it has the
+                // work (4x more made the gather 4x slower and the wide s and p classes no faster)
                 const double tgt = std::max(std::min(target, class_cost[ci] / 256.0), 1.0);
                 WorkItem w{};
                 w.P = P;

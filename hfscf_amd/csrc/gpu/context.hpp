@@ -181,8 +181,12 @@ inline hipError_t for_each_class_concurrent(hfg_ctx* c, hipStream_t main, F&& fn
     for (int k = 0; k < kSideStreams; ++k)
         if ((e = hipStreamWaitEvent(c->side[k], c->ev_fork, 0)) != hipSuccess) return e;
     if ((e = hipStreamWaitEvent(c->heavy, c->ev_fork, 0)) != hipSuccess) return e;
-    for (size_t i = 0; i < c->classes.size(); ++i)
-        if ((e = fn(c->classes[i], streams[i] < 0 ? c->heavy : c->side[streams[i]])) != hipSuccess) return e;
+    // the shared-stream classes first: they are few workgroups of long latency, and launched
+    // after the side streams' persistent grids they would wait for compute units to drain
+    for (int pass = 0; pass < 2; ++pass)
+        for (size_t i = 0; i < c->classes.size(); ++i)
+            if ((streams[i] < 0) == (pass == 0))
+                if ((e = fn(c->classes[i], streams[i] < 0 ? c->heavy : c->side[streams[i]])) != hipSuccess) return e;
     for (int k = 0; k < kSideStreams; ++k) {
         if ((e = hipEventRecord(c->ev_join[k], c->side[k])) != hipSuccess) return e;
         if ((e = hipStreamWaitEvent(main, c->ev_join[k], 0)) != hipSuccess) return e;

@@ -103,4 +103,6 @@ def test_batch_rhf_matches_solo(precision):
               f"vs {solo['iterations']}", flush=True)
         assert res[k]["converged"] and solo["converged"]
         assert abs(de) <= 1e-9
-        assert dg <= 1e-8
+        # FP64: the same numbers. Mixed: the FP32 gradient kernels evaluated at two densities that
+        # agree to the SCF tolerance differ by their FP32 rounding, about 1e-7 Eh/bohr
+        assert dg <= (1e-8 if precision == H.FP64 else 2e-6)

@@ -92,9 +92,13 @@ def test_mixed_scf_c12h26():
     # precision-mode workload: C12H26 / 6-31G** (Cartesian d, 310 functions)
     mol = _mol("c12h26.xyz", "6-31g**")
     ref = H.rhf(mol, diis_range=8)
-    mix = H.rhf(mol, diis_range=8, precision=H.MIXED, incremental=1)
+    # FP32 density-difference builds leave an error of order 1e-8 in the accumulated Fock matrix,
+    # which bounds how far the DIIS error can fall: the mixed run stops at rms 1e-7 and its
+    # energy is that of the final FP64 build
+    mix = H.rhf(mol, diis_range=8, precision=H.MIXED, incremental=1, energy_tol=1e-9, rms_tol=1e-7)
     de = mix["energy_final"] - ref["energy_final"]
-    print(f"C12H26: E {ref['energy_final']:.10f}, mixed dE {de:.2e}", flush=True)
+    print(f"C12H26: E {ref['energy_final']:.10f} ({ref['iterations']} iterations, converged {ref['converged']}), "
+          f"mixed dE {de:.2e} ({mix['iterations']} iterations, converged {mix['converged']})", flush=True)
     assert mix["converged"] and ref["converged"]
     assert abs(de) <= 1e-6
     path = os.path.join(GOLD, "cfg3_c12h26_631gss.npz")
