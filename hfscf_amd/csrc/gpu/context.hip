@@ -222,6 +222,38 @@ int setup_geometry(hfg_ctx* c) {
                 v.push_back(cp);
             }
     }
+    // Primitive pairs that cannot reach the primitive cutoff in any quartet: every kernel skips a
+    // primitive quartet when |K_ab K_cd| / sqrt(p + q) is below its cutoff (the J/K and gradient
+    // cut integral_tol/1e4, the Schwarz diagonal 1e-16), and |K_cd| / sqrt(p + q) <= Mq =
+    // max |K| / sqrt(q) over all primitive pairs, so a primitive pair with |K_ab| Mq below the
+    // smaller cutoff contributes to no quartet. Such primitive pairs are moved behind the others
+    // (which keep their basis order: the FP32 kernels' primitive sums are sensitive to the order,
+    // and a largest-first order made them 30 times less accurate) and the loop count np stops
+    // before them; the storage keeps all of them.
+    {
+        double mq = 0.0;
+        for (size_t k = 0; k < pp0.size(); ++k) mq = std::max(mq, std::fabs(pp1[k].w) / std::sqrt(pp0[k].x));
+        const double cut = std::min(1e-16, c->opt.integral_tol / 1e4);
+        std::vector<int> perm;
+        std::vector<double4> t0, t1, t2;
+        for (auto& v : cand)
+            for (auto& cp : v) {
+                const int n = c->nprim[cp.ga] * c->nprim[cp.gb], o = cp.p.off;
+                perm.resize(n);
+                std::iota(perm.begin(), perm.end(), 0);
+                auto sig = [&](int x) { return std::fabs(pp1[o + x].w) * mq >= cut; };
+                const int nsig = (int)(std::stable_partition(perm.begin(), perm.end(), sig) - perm.begin());
+                t0.assign(pp0.begin() + o, pp0.begin() + o + n);
+                t1.assign(pp1.begin() + o, pp1.begin() + o + n);
+                t2.assign(pp2.begin() + o, pp2.begin() + o + n);
+                for (int k = 0; k < n; ++k) {
+                    pp0[o + k] = t0[perm[k]];
+                    pp1[o + k] = t1[perm[k]];
+                    pp2[o + k] = t2[perm[k]];
+                }
+                cp.p.np = nsig;
+            }
+    }
     HFG_CHECK(c, c->d_pp0.upload(pp0, c->stream));
     HFG_CHECK(c, c->d_pp1.upload(pp1, c->stream));
     HFG_CHECK(c, c->d_pp2.upload(pp2, c->stream));
@@ -304,6 +336,7 @@ int setup_geometry(hfg_ctx* c) {
             lists[g].push_back({s, &cp});
         }
     c->pairs.clear();
+    std::vector<int> pair_nfull;  // stored primitive pairs of each pair (np counts the significant ones)
     c->psmean.clear();
     c->psq.clear();
     c->bp_off.assign(nbp, 0);
@@ -329,6 +362,7 @@ int setup_geometry(hfg_ctx* c) {
                 pd.sq_max = std::max(pd.sq_max, q);
             }
             c->pairs.push_back(pd);
+            pair_nfull.push_back(c->nprim[e.second->ga] * c->nprim[e.second->gb]);
             c->psmean.push_back(e.first);
         }
         c->bp_cnt[g] = (int)c->pairs.size() - c->bp_off[g];
@@ -336,17 +370,18 @@ int setup_geometry(hfg_ctx* c) {
     // primitive data of the surviving pairs, primitive-major within each block pair's list
     {
         size_t tot = 0;
-        for (const auto& pd : c->pairs) tot += pd.np;
+        for (int nf : pair_nfull) tot += nf;
         std::vector<double4> q0(std::max<size_t>(tot, 1)), q1(q0.size()), q2(q0.size());
         size_t pos = 0;
         for (int g = 0; g < nbp; ++g) {
             const int cnt = c->bp_cnt[g];
             if (!cnt) continue;
             PairDev* lst = &c->pairs[c->bp_off[g]];
+            const int* nfl = &pair_nfull[c->bp_off[g]];
             bool uniform = true;
-            for (int j = 1; j < cnt; ++j) uniform = uniform && lst[j].np == lst[0].np;
+            for (int j = 1; j < cnt; ++j) uniform = uniform && nfl[j] == nfl[0];
             if (uniform) {
-                const int np = lst[0].np;
+                const int np = nfl[0];
                 for (int j = 0; j < cnt; ++j) {
                     for (int k = 0; k < np; ++k) {
                         q0[pos + (size_t)k * cnt + j] = pp0[lst[j].off + k];
@@ -359,14 +394,14 @@ int setup_geometry(hfg_ctx* c) {
                 pos += (size_t)np * cnt;
             } else {
                 for (int j = 0; j < cnt; ++j) {
-                    for (int k = 0; k < lst[j].np; ++k) {
+                    for (int k = 0; k < nfl[j]; ++k) {
                         q0[pos + k] = pp0[lst[j].off + k];
                         q1[pos + k] = pp1[lst[j].off + k];
                         q2[pos + k] = pp2[lst[j].off + k];
                     }
                     lst[j].off = (int)pos;
                     lst[j].pstride = 1;
-                    pos += lst[j].np;
+                    pos += nfl[j];
                 }
             }
         }
