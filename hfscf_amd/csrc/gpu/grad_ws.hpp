@@ -74,8 +74,11 @@ struct VrrTab {
     }
 };
 
-// VRR into the compact table (the recursion of Vrr::run, limited to |e| + |f| <= LTM).
-template <int LAB, int LCD, int LTM, typename R>
+// VRR into the compact table (the recursion of Vrr::run, limited to |e| + |f| <= LTM). The second
+// recursion has a fence after every f (one global-memory round trip per f); FINE puts one after
+// every (e, f) block instead, as the (dd|dd) class is built (its build takes two hours and was
+// not repeated with the coarser fences).
+template <int LAB, int LCD, int LTM, typename R, bool FINE>
 HFG_INLINE void vrr_ws(const R* F, const R* PA, const R* WP, const R* QC, const R* WQ, R o2p, R rp, R o2q, R rq,
                        R o2pq, WsView<R> v) {
     using T = VrrTab<LAB, LCD, LTM>;
@@ -121,9 +124,10 @@ HFG_INLINE void vrr_ws(const R* F, const R* PA, const R* WP, const R* QC, const 
                     }
                     v[o + m] = t;
                 });
-                ws_fence(v);
+                if constexpr (FINE) ws_fence(v);
             }
         });
+        if constexpr (!FINE) ws_fence(v);
     });
 }
 
@@ -426,7 +430,7 @@ __global__ __launch_bounds__(kWsBlock) void grad_ws_kernel(GradArgs a, double* _
                     const R QC[3] = {b1.x, b1.y, b1.z};
                     const R WP[3] = {-rp * PQx, -rp * PQy, -rp * PQz};
                     const R WQ[3] = {rq * PQx, rq * PQy, rq * PQz};
-                    vrr_ws<LA + LB + 1, LC + LD + 1, LT + 1, R>(F, PA, WP, QC, WQ, a2.x, rp, b2.x, rq, 0.5 * ipq, V);
+                    vrr_ws<LA + LB + 1, LC + LD + 1, LT + 1, R, (LT >= 8)>(F, PA, WP, QC, WQ, a2.x, rp, b2.x, rq, 0.5 * ipq, V);
                     const R wa = 2.0 * a2.y, wb = 2.0 * a2.z, wc = 2.0 * b2.y;
                     R s[3];
                     shift_contract_ws<LA, LB, LC, LD, 0, 1, T, R>(V, G, Bra, Hb, Hk, AB, CD, s);
