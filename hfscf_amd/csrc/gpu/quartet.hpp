@@ -52,16 +52,16 @@ __device__ __forceinline__ void boys_eval_rs(double T, const double* __restrict_
 template <int M, typename R>
 __device__ __forceinline__ void boys_low(double T, double scale, R* F) {
     static_assert(M <= 1, "closed form for F_0 and F_1 only");
-    double f0 = 1.0;
+    double f0 = 1.0, ri = 0.0;
     if (T > 0.0) {
-        const double ri = rsqrt(T);
+        ri = rsqrt(T);
         f0 = 0.88622692545275801365 * erf(T * ri) * ri;  // sqrt(pi)/2
     }
     F[0] = (R)(scale * f0);
     if constexpr (M == 1) {
         double f1;
         if (T >= 0.05) {
-            f1 = (f0 - exp(-T)) * (0.5 / T);
+            f1 = (f0 - exp(-T)) * (0.5 * ri * ri);  // 1/T from the square root already taken
         } else {
             f1 = 1.0 / 766080.0;
             f1 = -1.0 / 85680.0 + T * f1;
