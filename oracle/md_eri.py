@@ -111,18 +111,30 @@ def primitive_eri(la, A, a, lb, B, b, lc, C, c, ld, D, d):
     return out
 
 
-def contracted_eri(shells, xyz, s1, s2, s3, s4):
-    """Contracted Cartesian block using the normalised coefficients in shells['coef']."""
+def contracted_eri(shells, xyz, s1, s2, s3, s4, prim_cut=0.0):
+    """Contracted Cartesian block using the normalised coefficients in shells['coef'].
+
+    prim_cut > 0 drops primitive quartets whose prefactor
+    |c_i c_j c_k c_l e^{-ab/p AB^2} e^{-cd/q CD^2} 2 pi^{5/2} / (p q sqrt(p+q))| is below it."""
     po = np.concatenate([[0], np.cumsum(shells["nprim"])])
     L = shells["L"]
     cen = [np.asarray(xyz[shells["atom"][s]], dtype=float) for s in (s1, s2, s3, s4)]
-    res = None
+    al = shells["alpha"]
+    ab2 = float(np.sum((cen[0] - cen[1]) ** 2))
+    cd2 = float(np.sum((cen[2] - cen[3]) ** 2))
+    res = np.zeros([len(comps(L[s])) for s in (s1, s2, s3, s4)])
     for i in range(po[s1], po[s1 + 1]):
         for j in range(po[s2], po[s2 + 1]):
             for k in range(po[s3], po[s3 + 1]):
                 for l in range(po[s4], po[s4 + 1]):
                     w = shells["coef"][i] * shells["coef"][j] * shells["coef"][k] * shells["coef"][l]
+                    if prim_cut > 0.0:
+                        p, q = al[i] + al[j], al[k] + al[l]
+                        pf = w * math.exp(-al[i] * al[j] / p * ab2 - al[k] * al[l] / q * cd2) * 2 * math.pi ** 2.5 / (
+                            p * q * math.sqrt(p + q))
+                        if abs(pf) < prim_cut:
+                            continue
                     blk = w * primitive_eri(L[s1], cen[0], shells["alpha"][i], L[s2], cen[1], shells["alpha"][j],
                                             L[s3], cen[2], shells["alpha"][k], L[s4], cen[3], shells["alpha"][l])
-                    res = blk if res is None else res + blk
+                    res = res + blk
     return res

@@ -1,12 +1,13 @@
 """Extract the H, C and O entries of four Psi4-format (.gbs) basis sets.
 
 The basis-set data (Basis Set Exchange, public) are read from the hfscf
-checkout (``basis/<name>.gbs``) and written next to this script with the
+checkout (``basis/<name>.gbs``) and written to the package's basis folder
+(``hfscf_amd/basis``) with the
 header line (``spherical``/``cartesian``) and the element blocks kept exactly
 as published, so shell order, SP blocks and contraction coefficients are the
 ones hfscf itself reads (hfscf: src/basis/hfscf_basis.cpp:19-242).
 
-Usage: python hfscf_amd/basis/extract_basis.py /path/to/hfscf
+Usage: python tests/golden/extract_basis.py /path/to/hfscf
 """
 import os
 import sys
@@ -38,8 +39,9 @@ def extract(src, dst):
 def main():
     root = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
     here = os.path.dirname(os.path.abspath(__file__))
+    dest = os.path.join(os.path.dirname(os.path.dirname(here)), "hfscf_amd", "basis")
     for n in NAMES:
-        extract(os.path.join(root, "basis", n + ".gbs"), os.path.join(here, n + ".gbs"))
+        extract(os.path.join(root, "basis", n + ".gbs"), os.path.join(dest, n + ".gbs"))
 
 
 if __name__ == "__main__":

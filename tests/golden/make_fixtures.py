@@ -9,7 +9,8 @@ Each fixture is a small .npz of inputs and expected outputs produced by the orac
                          random density
   cfg3_c12h26_631gss.npz the same for C12H26 / 6-31G**
   grad_h2o.npz           two-electron and total RHF gradients of water (3 bases)
-  grad_h2o16.npz         (H2O)16 / cc-pVDZ converged energy and analytic RHF gradient
+  energy_h2o16.npz       (H2O)16 / cc-pVDZ converged energy, from the oracle alone
+  grad_h2o16.npz         the same and the analytic RHF gradient (total and two-electron part)
 """
 import os
 import sys
@@ -88,23 +89,43 @@ def grad():
 
 
 def h2o16():
-    # (H2O)16 / cc-pVDZ: converged SCF (stored-ERI oracle, about 22 GB of ERIs) and the oracle's
-    # analytic RHF gradient at that density (on-the-fly derivative ERIs)
-    mol = H.Molecule.from_xyz(os.path.join(HERE, "h2o_16.xyz"), "cc-pvdz")
-    orc = oracle.Oracle(mol.shells(), mol.geometry()[1], mol.pure, 1e-14, screen=True)
+    """(H2O)16 / cc-pVDZ from the oracle alone (no product code): basis and ordering
+    (oracle/basis_ref.py), one-electron integrals and gradient terms (oracle/oneint_ref.py),
+    stored ERIs and the two-electron gradient (ref_oracle.cpp), hfscf's SCF loop
+    (oracle/scf_ref.py). About 22 GB of stored ERIs; the analytic gradient takes hours of CPU."""
+    from oracle import basis_ref, oneint_ref, scf_ref
+
+    Z, X = basis_ref.read_xyz(os.path.join(HERE, "h2o_16.xyz"))
+    sh, pure, n = basis_ref.shells_for(Z, "cc-pvdz")
+    t = time.time()
+    S, T, V = oneint_ref.oneint(sh, pure, Z, X)
+    Hc = T + V
+    enuc = oneint_ref.enuc(Z, X)
+    print("h2o16 oneint", time.time() - t, flush=True)
+    orc = oracle.Oracle(sh, X, pure, 1e-14, screen=True)
     t = time.time()
     orc.build_eri()
     print("h2o16 eri build", time.time() - t, flush=True)
-    r = H.rhf(mol, jk=orc.jk, diis_range=8, energy_tol=1e-12, rms_tol=1e-10)
-    print("h2o16 energy", r["energy"], r["energy_final"], r["iterations"], flush=True)
+    r = scf_ref.rhf(S, Hc, int(sum(Z)) // 2, orc.jk, diis_range=8, energy_tol=1e-12, rms_tol=1e-11)
+    D = r["D"]
+    J, K = orc.jk(D)
+    F = Hc + 2.0 * J - K  # Fock matrix of the converged density
+    energy = r["energy_elec"] + enuc
+    energy_final = float(np.sum(D * (Hc + F))) + enuc
+    print("h2o16 energy", energy, energy_final, r["iterations"], r["converged"], flush=True)
     # the energies first (the analytic gradient below takes hours of CPU)
-    np.savez_compressed(os.path.join(HERE, "energy_h2o16.npz"), energy=r["energy"], energy_final=r["energy_final"],
+    np.savez_compressed(os.path.join(HERE, "energy_h2o16.npz"), energy=energy, energy_final=energy_final,
                         iterations=r["iterations"])
+    del orc  # frees the stored ERIs before the on-the-fly derivative integrals
+    orc = oracle.Oracle(sh, X, pure, 1e-14, screen=True)
     t = time.time()
-    g = H.rhf_gradient(mol, r["D"], r["F"], jk=orc.jk, grad2e=orc.grad2e)
-    print("h2o16 gradient", time.time() - t, np.abs(g.sum(0)).max(), flush=True)
-    np.savez_compressed(os.path.join(HERE, "grad_h2o16.npz"), energy=r["energy"], energy_final=r["energy_final"],
-                        iterations=r["iterations"], g=g)
+    g1 = oneint_ref.oneint_gradient(sh, pure, Z, X, D, D @ F @ D) + oneint_ref.enuc_gradient(Z, X)
+    print("h2o16 one-electron gradient", time.time() - t, flush=True)
+    t = time.time()
+    g2 = orc.grad2e(D)
+    print("h2o16 two-electron gradient", time.time() - t, np.abs((g1 + g2).sum(0)).max(), flush=True)
+    np.savez_compressed(os.path.join(HERE, "grad_h2o16.npz"), energy=energy, energy_final=energy_final,
+                        iterations=r["iterations"], g=g1 + g2, g2e=g2)
 
 
 if __name__ == "__main__":
