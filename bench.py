@@ -2,18 +2,21 @@
 
 A step is one direct Fock build F = H + 2J - K (every surviving shell quartet evaluated,
 digested and summed on the GPU) for the converged density of the workload. Default workload:
-(H2O)8 / cc-pVDZ (pure d, 192 basis functions), Schwarz threshold 1e-12, FP64.
+(H2O)16 / cc-pVDZ (pure d, 384 basis functions, 48 atoms), Schwarz threshold 1e-12, FP64.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload h2o8|c12h26|h2o16]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload h2o16|h2o8|c12h26]
                     [--no-cpu] [--no-extra] [--no-mixed] [--no-scf] [--batch-molecules N]
 
 Prints one JSON line (value = FP64 Fock builds/s on the GPU) with a roofline object for the
-J/K kernels (model FLOPs of the surviving quartets / HIP-event kernel time, against the FP64
-vector peak) and the CPU baseline (the oracle, timed on a bounded sample on this host).
-"extra" holds the other figures of the metric: two-electron gradients/s (adjoint kernel),
-the mixed-precision Fock build and gradient rates at the same density with their errors
-against FP64, whole-SCF timings of the FP64 and mixed (incremental) solvers with |dE|, and the
-batching mode (512 seeded water clusters in one lockstep SCF) against one-at-a-time solves.
+dominant J/K kernel (model FLOPs of its surviving quartets / its HIP-event duration, against
+the FP64 vector peak; HBM traffic from the committed rocprofv3 counter summary when it names
+the same kernel and workload) and the CPU baseline (the oracle, timed on a bounded sample on
+this host). "extra" holds the other figures of the metric on the same workload: FP64 and
+mixed-precision Fock builds/s and gradients/s at the same density with their errors,
+like-for-like SCF runs (FP64 full builds as hfscf does, FP64 density-difference builds, mixed
+density-difference builds with an FP64 finish, all to the same stopping rule) with |dE| and the
+end-to-end gradient error, and the batching mode (512 seeded water clusters in one lockstep SCF)
+against one-at-a-time solves. No part of the run imports torch.
 """
 import argparse
 import json
@@ -34,8 +37,11 @@ PEAK_FP32 = 157.3  # TFLOP/s, MI355X FP32 vector (packed; 78.6 without packed FP
 WORKLOADS = {
     "h2o8": ("h2o_8.xyz", "cc-pvdz", "(H2O)8 / cc-pVDZ (pure d), 192 bf, direct RHF Fock build, Schwarz 1e-12"),
     "c12h26": ("c12h26.xyz", "6-31g**", "C12H26 / 6-31G** (Cartesian d), 310 bf, direct RHF Fock build"),
-    "h2o16": ("h2o_16.xyz", "cc-pvdz", "(H2O)16 / cc-pVDZ (pure d), 384 bf, direct RHF Fock build"),
+    "h2o16": ("h2o_16.xyz", "cc-pvdz", "(H2O)16 / cc-pVDZ (pure d), 384 bf, 48 atoms, direct RHF Fock build, "
+                                        "Schwarz 1e-12"),
 }
+ORACLE_ENERGY = {"h2o8": "cfg2_h2o8_ccpvdz.npz", "h2o16": "energy_h2o16.npz"}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "jk_traffic.json")
 SCF_OPTS = dict(diis_range=8, energy_tol=1e-10, rms_tol=1e-10)
 
 
@@ -67,32 +73,34 @@ def time_grads(ctx, D, steps, warmup):
     return time.perf_counter() - t0, kern, g
 
 
-def copy_calibration(mib, reps=3):
-    """A plain device-to-device copy of a known size, so that a profiler's HBM byte counters can
-    be scaled against it."""
-    import torch
-
-    n = int(mib * 2 ** 20) // 4
-    a = torch.ones(n, dtype=torch.float32, device="cuda")
-    b = torch.empty_like(a)
-    for _ in range(reps):
-        b.copy_(a)
-    torch.cuda.synchronize()
-    log(f"copy calibration: {reps} copies of {n * 4} bytes (reads {n * 4} B, writes {n * 4} B each)")
+def cpu_model():
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
-def cpu_baseline(mol, budget_rows=32):
-    """Oracle (CPU restatement of hfscf's stored-ERI path) on this host: ERI rows sampled with a
-    fixed stride (an unbiased estimate of the full ERI build) plus one full digestion."""
+def cpu_baseline(mol, rows=6, threads=None):
+    """The oracle (CPU restatement of hfscf's stored-ERI path, OpenMP) on this host's cores: the ERI
+    build of `rows` shell rows taken at the midpoints of equal strata of the shell index (rows
+    hold unequal work, so the midpoint of each stratum, not its first row), scaled by the stride,
+    plus one full stored-ERI digestion. Also one row on a single thread against the same row on
+    all threads (median of three each), the ratio giving the one-thread figure."""
     import oracle
 
     lib = oracle.lib()
+    nthreads = threads or os.cpu_count() or 1
+    lib.orc_set_threads(nthreads)
     orc = oracle.Oracle(mol.shells(), mol.geometry()[1], mol.pure, 1e-14, screen=True)
     ns = mol.nshell
-    stride = max(1, ns // budget_rows)
+    stride = max(1, ns // rows)
+    picked = list(range(stride // 2, ns, stride))
     t_rows = 0.0
     q = 0
-    for r in range(0, ns, stride):
+    for r in picked:
         t = time.perf_counter()
         q += lib.orc_build_eri_rows(orc.h, r, r + 1)
         t_rows += time.perf_counter() - t
@@ -100,94 +108,135 @@ def cpu_baseline(mol, budget_rows=32):
     orc._built = True
     n = mol.nbf
     D = np.eye(n) * 0.5
-    Hc = np.zeros((n, n))
     t = time.perf_counter()
-    orc.fock(Hc, D)
+    orc.fock(np.zeros((n, n)), D)
     t_dig = time.perf_counter() - t
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    # single-thread figure: one small row, 1 thread against all threads
+    r1 = picked[0]
+
+    def med(k):
+        lib.orc_set_threads(k)
+        ts = []
+        for _ in range(3):
+            t = time.perf_counter()
+            lib.orc_build_eri_rows(orc.h, r1, r1 + 1)
+            ts.append(time.perf_counter() - t)
+        return float(np.median(ts))
+
+    t1, tn = med(1), med(nthreads)
+    lib.orc_set_threads(nthreads)
+    del orc
+    value = 1.0 / (t_eri + t_dig)
     return {
-        "value": 1.0 / (t_eri + t_dig),
+        "value": value,
         "unit": "Fock builds/s",
-        "cores": threads,
+        "cores": nthreads,
         "kind": "port",
-        "sample": f"oracle ERI build on shell rows 0..{ns - 1} step {stride} ({q} shell quartets, "
-                  f"{t_rows:.2f} s, scaled x{stride} to {t_eri:.1f} s) + one full stored-ERI digestion "
-                  f"({t_dig:.2f} s), {threads} OpenMP threads",
+        "cpu_model": cpu_model(),
+        "value_1_thread": value * tn / t1,
+        "sample": f"oracle ERI build of {len(picked)} of {ns} shell rows (rows {picked}, stride {stride}, {q} shell "
+                  f"quartets, {t_rows:.2f} s, scaled x{stride} to {t_eri:.1f} s) + one full stored-ERI digestion "
+                  f"({t_dig:.2f} s) on {nthreads} OpenMP threads; one thread: row {r1} takes {t1:.2f} s against "
+                  f"{tn:.3f} s on {nthreads} threads (median of 3), value_1_thread = value x {tn / t1:.4f}",
     }
 
 
-def dominant_roofline(classes):
+def load_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` on `workload` from the committed rocprofv3 counter summary
+    (FETCH_SIZE + WRITE_SIZE, scaled by the same counters on a copy kernel of known size)."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None, None
+    e = t.get(workload, {}).get(kernel)
+    if not e:
+        return None, None
+    return e.get("bytes_per_launch"), e
+
+
+def dominant_roofline(classes, workload):
     """Roofline of the dominant J/K kernel (the class launch with the longest duration in the last
-    build): model FLOPs of its surviving quartets / its HIP-event duration, against FP64 vector peak.
-    Traffic is None: the profiler's HBM byte counters could not be collected for these kernels."""
+    build): model FLOPs of its surviving quartets / its HIP-event duration, against FP64 vector peak."""
     c = max(classes, key=lambda c: c["kernel_ms"])
+    cls = "".join(str(x) for x in c["cls"])
     ach = c["model_flops"] / (c["kernel_ms"] * 1e-3) / 1e12
-    return {"bound": "valu", "kernel": f"jk_kernel ({c['cls'][0]}{c['cls'][1]}|{c['cls'][2]}{c['cls'][3]}) fp64",
-            "achieved": ach, "peak": PEAK_FP64, "unit": "TFLOP/s", "frac": ach / PEAK_FP64, "traffic": None,
-            "model_gflop_per_launch": c["model_flops"] / 1e9, "kernel_ms": c["kernel_ms"]}
+    traffic, entry = load_traffic(f"jk_kernel_{cls}_fp64", workload)
+    out = {"bound": "valu", "kernel": f"jk_kernel ({cls[:2]}|{cls[2:]}) fp64", "achieved": ach, "peak": PEAK_FP64,
+           "unit": "TFLOP/s", "frac": ach / PEAK_FP64, "traffic": traffic,
+           "model_gflop_per_launch": c["model_flops"] / 1e9, "kernel_ms": c["kernel_ms"],
+           "quartets_per_launch": c["quartets"]}
+    if entry:
+        out["traffic_source"] = entry.get("source")
+        out["arithmetic_intensity_flop_per_byte"] = c["model_flops"] / traffic if traffic else None
+    return out
 
 
 def gradient_figures(mol, D, steps, warmup, autotune=True):
-    """FP64 two-electron gradients/s (adjoint kernel) at the workload's density."""
-    ctx = H.GpuContext(mol)
-    if autotune:
-        ctx.autotune(2)
-    el, kern, g = time_grads(ctx, D, steps, warmup)
-    st = ctx.stats()
-    ctx.close()
-    log(f"FP64 gradient: {steps / el:.2f}/s, kernel {1e3 * kern / steps:.2f} ms")
-    return {"main_fp64_gradients_per_s": steps / el, "main_fp64_gradient_kernel_ms": 1e3 * kern / steps,
-            "main_fp64_gradient_model_tflops": st["model_flops"] / (kern / steps) / 1e12}
-
-
-def mixed_figures(workload, steps, warmup):
-    """Mixed precision against FP64 on one workload ((H2O)16 / cc-pVDZ by default): Fock builds/s
-    and gradients/s at the converged FP64 density (full builds, no density difference), and whole
-    SCF runs, FP64 with full builds as hfscf does and mixed with FP32 density-difference builds,
-    density screening and a final FP64 build, with |dE| and the gradient error."""
-    xyz, basis, desc = WORKLOADS[workload]
-    mol = H.Molecule.from_xyz(os.path.join(ROOT, "tests", "golden", xyz), basis)
-    S, T, V = mol.oneint()
-    Hc = T + V
-    out = {"mixed_workload": desc}
-    t0 = time.perf_counter()
-    scf64 = H.rhf(mol, **SCF_OPTS)
-    out["scf_fp64_seconds"] = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    # FP32 density-difference builds leave about 1e-8 of error in the accumulated Fock matrix, so
-    # the mixed SCF stops at rms 1e-7 (its energy is the final FP64 build's)
-    scfm = H.rhf(mol, precision=H.MIXED, incremental=1, **dict(SCF_OPTS, energy_tol=1e-9, rms_tol=1e-7))
-    out["scf_mixed_seconds"] = time.perf_counter() - t0
-    out["scf_fp64_fock_seconds"] = scf64["fock_seconds"]
-    out["scf_mixed_fock_seconds"] = scfm["fock_seconds"]
-    out["scf_fp64_iterations"] = scf64["iterations"]
-    out["scf_mixed_iterations"] = scfm["iterations"]
-    out["scf_fp64_converged"] = bool(scf64["converged"])
-    out["scf_mixed_converged"] = bool(scfm["converged"])
-    out["scf_mixed_dE_vs_fp64"] = scfm["energy_final"] - scf64["energy_final"]
-    out["scf_fock_speedup_mixed_vs_fp64"] = scf64["fock_seconds"] / max(scfm["fock_seconds"], 1e-12)
-    log(f"{workload} SCF FP64 {out['scf_fp64_seconds']:.2f} s (Fock {scf64['fock_seconds']:.2f} s), mixed "
-        f"{out['scf_mixed_seconds']:.2f} s (Fock {scfm['fock_seconds']:.2f} s), dE {out['scf_mixed_dE_vs_fp64']:.2e}")
-    D = scf64["D"]
+    """FP64 and mixed two-electron gradients/s (adjoint kernel) at the workload's density."""
+    out = {}
     res = {}
     for tag, prec in [("fp64", H.FP64), ("mixed", H.MIXED)]:
         ctx = H.GpuContext(mol, precision=prec)
-        el, kern = time_builds(ctx, Hc, D, steps, warmup)
-        F = ctx.fock(Hc, D)
-        elg, kerng, g = time_grads(ctx, D, steps, warmup)
+        if autotune:
+            ctx.autotune(2)
+        el, kern, g = time_grads(ctx, D, steps, warmup)
+        st = ctx.stats()
         ctx.close()
-        res[tag] = (F, g)
+        res[tag] = g
+        out[f"{tag}_gradients_per_s"] = steps / el
+        out[f"{tag}_gradient_kernel_ms"] = 1e3 * kern / steps
+        out[f"{tag}_gradient_model_tflops"] = st["model_flops"] / (kern / steps) / 1e12
+        log(f"{tag} gradient: {steps / el:.2f}/s, kernel {1e3 * kern / steps:.2f} ms")
+    out["mixed_gradient_kernel_max_abs_err_vs_fp64"] = float(np.abs(res["mixed"] - res["fp64"]).max())
+    out["gradient_speedup_mixed_vs_fp64"] = out["mixed_gradients_per_s"] / out["fp64_gradients_per_s"]
+    return out
+
+
+def mixed_build_figures(mol, Hc, D, steps, warmup, autotune=True):
+    """Mixed-precision full Fock builds/s at the same density as the FP64 headline, and the error
+    of the mixed Fock matrix and energy there."""
+    out = {}
+    F = {}
+    for tag, prec in [("fp64", H.FP64), ("mixed", H.MIXED)]:
+        ctx = H.GpuContext(mol, precision=prec)
+        if autotune:
+            ctx.autotune(1)
+        el, kern = time_builds(ctx, Hc, D, steps, warmup)
+        F[tag] = ctx.fock(Hc, D)
+        ctx.close()
         out[f"{tag}_fock_builds_per_s"] = steps / el
         out[f"{tag}_fock_kernel_ms"] = 1e3 * kern / steps
-        out[f"{tag}_gradients_per_s"] = steps / elg
-        out[f"{tag}_gradient_kernel_ms"] = 1e3 * kerng / steps
-    out["mixed_fock_max_abs_err_vs_fp64"] = float(np.abs(res["mixed"][0] - res["fp64"][0]).max())
-    out["mixed_energy_err_at_fixed_D"] = float(np.sum(D * (res["mixed"][0] - res["fp64"][0])))
-    out["mixed_gradient_max_abs_err_vs_fp64"] = float(np.abs(res["mixed"][1] - res["fp64"][1]).max())
+    out["mixed_fock_max_abs_err_vs_fp64"] = float(np.abs(F["mixed"] - F["fp64"]).max())
+    out["mixed_energy_err_at_fixed_D"] = float(np.sum(D * (F["mixed"] - F["fp64"])))
     out["fock_speedup_mixed_vs_fp64"] = out["mixed_fock_builds_per_s"] / out["fp64_fock_builds_per_s"]
-    out["gradient_speedup_mixed_vs_fp64"] = out["mixed_gradients_per_s"] / out["fp64_gradients_per_s"]
-    log(f"{workload} builds/s FP64 {out['fp64_fock_builds_per_s']:.2f} mixed {out['mixed_fock_builds_per_s']:.2f}; "
-        f"gradients/s FP64 {out['fp64_gradients_per_s']:.2f} mixed {out['mixed_gradients_per_s']:.2f}")
+    log(f"builds/s FP64 {out['fp64_fock_builds_per_s']:.2f} mixed {out['mixed_fock_builds_per_s']:.2f}")
+    return out
+
+
+def scf_figures(mol, scf64, g64):
+    """Like-for-like SCF runs to the same stopping rule (energy 1e-10, rms 1e-10): FP64 with full
+    builds (hfscf's scheme, `scf64`, already run), FP64 with density-difference builds and density
+    screening, and mixed precision (FP32 density-difference builds, FP64 finish from rms 1e-6);
+    |dE| of each against the first and the end-to-end gradient error of the mixed run."""
+    out = {"scf_fp64_full_seconds": scf64["seconds"], "scf_fp64_full_fock_seconds": scf64["fock_seconds"],
+           "scf_fp64_full_iterations": scf64["iterations"], "scf_fp64_full_converged": bool(scf64["converged"])}
+    for tag, kw in [("fp64_incremental", dict(incremental=1)), ("mixed", dict(precision=H.MIXED, incremental=1))]:
+        t0 = time.perf_counter()
+        r = H.rhf(mol, **dict(SCF_OPTS, **kw))
+        out[f"scf_{tag}_seconds"] = time.perf_counter() - t0
+        out[f"scf_{tag}_fock_seconds"] = r["fock_seconds"]
+        out[f"scf_{tag}_iterations"] = r["iterations"]
+        out[f"scf_{tag}_converged"] = bool(r["converged"])
+        out[f"scf_{tag}_dE_vs_fp64_full"] = r["energy_final"] - scf64["energy_final"]
+        out[f"scf_fock_speedup_{tag}_vs_fp64_full"] = scf64["fock_seconds"] / max(r["fock_seconds"], 1e-12)
+        if tag == "mixed":
+            gm = H.rhf_gradient(mol, r["D"], r["F"], precision=H.MIXED)
+            out["scf_mixed_gradient_max_abs_err_vs_fp64"] = float(np.abs(gm - g64).max())
+            inc = out["scf_fp64_incremental_fock_seconds"]
+            out["scf_fock_speedup_mixed_vs_fp64_incremental"] = inc / max(r["fock_seconds"], 1e-12)
+        log(f"SCF {tag}: {out[f'scf_{tag}_seconds']:.2f} s (Fock {r['fock_seconds']:.2f} s, {r['iterations']} it), "
+            f"dE {out[f'scf_{tag}_dE_vs_fp64_full']:.2e}")
     return out
 
 
@@ -216,7 +265,8 @@ def batch_figures(count, sample, seed=11):
     log(f"solo sample of {len(idx)}: {solo_rate:.2f} molecules/s, max |dE| {de:.2e}, max |dg| {dg:.2e}")
     nconv = sum(r["converged"] for r in res)
     return {"batch_molecules": len(mols), "batch_converged": nconv, "batch_seconds": tb,
-            "batch_molecules_per_s": len(mols) / tb, "batch_iterations": st["iterations"],
+            "batch_molecules_per_s": len(mols) / tb, "batch_converged_molecules_per_s": nconv / tb,
+            "batch_iterations": st["iterations"],
             "batch_ms_per_iteration": 1e3 * st["scf_seconds"] / max(st["iterations"], 1),
             "batch_setup_seconds": st["setup_seconds"], "batch_autotune_seconds": st["autotune_seconds"],
             "batch_gradient_seconds": st["gradient_seconds"], "batch_quartets_per_build": st["quartets"],
@@ -230,7 +280,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="h2o8", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="h2o16", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
@@ -240,31 +290,35 @@ def main():
                     help="time the builds on a seeded random density instead of the converged one "
                          "(FP64 build cost does not depend on the density values; for profiler runs)")
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="MIB",
-                    help="also run a plain device copy of this many MiB (HBM counter calibration)")
-    ap.add_argument("--mixed-workload", default="h2o16", choices=sorted(WORKLOADS),
-                    help="workload of the mixed-precision against FP64 figures")
+                    help="also run a plain copy kernel of this many MiB through libhfgpu (HBM counter "
+                         "calibration)")
     ap.add_argument("--batch-molecules", type=int, default=512,
                     help="batching-mode figures over this many seeded water clusters (0 = skip)")
     ap.add_argument("--solo-sample", type=int, default=24,
                     help="molecules solved one at a time for the batching comparison")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path (see DESIGN.md, sharding)",
-              file=sys.stderr)
+        print("bench.py runs on one GPU only: this build has no multi-GPU path (the design of the sharded "
+              "path is described in the design notes)", file=sys.stderr)
         sys.exit(2)
     xyz, basis, desc = WORKLOADS[args.workload]
     mol = H.Molecule.from_xyz(os.path.join(ROOT, "tests", "golden", xyz), basis)
     S, T, V = mol.oneint()
     Hc = T + V
     if args.copy_calibration > 0:
-        copy_calibration(args.copy_calibration)
+        ms = H.copy_calibration(args.copy_calibration)
+        nbytes = int(args.copy_calibration * 2 ** 20)
+        log(f"copy calibration: copy_kernel of {nbytes} bytes (reads {nbytes} B, writes {nbytes} B), "
+            f"{ms:.3f} ms per copy, {2 * nbytes / ms / 1e6:.1f} GB/s")
     if args.no_scf:
         rng = np.random.default_rng(7)
         A = rng.standard_normal((mol.nbf, mol.nbf))
         D = 0.05 * (A + A.T)
-        scf = dict(energy=float("nan"), energy_final=float("nan"), iterations=0, seconds=0.0, fock_seconds=0.0)
+        scf = dict(energy=float("nan"), energy_final=float("nan"), iterations=0, seconds=0.0, fock_seconds=0.0,
+                   converged=False)
     else:
-        # converged density from a GPU SCF (untimed)
+        # converged density from the GPU SCF with full FP64 builds (hfscf's scheme; untimed here,
+        # reported in extra)
         t0 = time.perf_counter()
         scf = H.rhf(mol, **SCF_OPTS)
         scf["seconds"] = time.perf_counter() - t0
@@ -295,12 +349,11 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": desc, "nbf": mol.nbf, "shell_quartets": st["quartets"], "precision": "fp64",
-                   "scf_energy": scf["energy"], "scf_energy_final": scf["energy_final"],
-                   "scf_iterations": scf["iterations"], "scf_seconds": scf["seconds"],
-                   "quartets_per_s": st["quartets"] * builds_s,
+        "config": {"workload": desc, "nbf": mol.nbf, "natom": mol.natom, "shell_quartets": st["quartets"],
+                   "precision": "fp64", "scf_energy": scf["energy"], "scf_energy_final": scf["energy_final"],
+                   "scf_iterations": scf["iterations"], "quartets_per_s": st["quartets"] * builds_s,
                    "launch_shapes": "default" if args.no_autotune else "autotuned"},
-        "roofline": dominant_roofline(classes),
+        "roofline": dominant_roofline(classes, args.workload),
         "build_roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP64, "unit": "TFLOP/s",
                            "frac": achieved / PEAK_FP64, "model_gflop_per_build": st["model_flops"] / 1e9,
                            "kernel_ms_per_build": 1e3 * kern_per,
@@ -310,18 +363,22 @@ def main():
                      "tflops": c["model_flops"] / c["kernel_ms"] / 1e9 if c["kernel_ms"] > 0 else None}
                     for c in sorted(classes, key=lambda c: -c["kernel_ms"])],
     }
-    ref_path = os.path.join(ROOT, "tests", "golden", "cfg2_h2o8_ccpvdz.npz")
-    if args.workload == "h2o8" and os.path.exists(ref_path):
-        f = np.load(ref_path)
-        out["config"]["dE_vs_oracle"] = scf["energy"] - float(f["energy"])
-        if "energy_final" in f:
-            out["config"]["dE_final_vs_oracle"] = scf["energy_final"] - float(f["energy_final"])
+    ref = ORACLE_ENERGY.get(args.workload)
+    if ref and not args.no_scf:
+        f = np.load(os.path.join(ROOT, "tests", "golden", ref))
+        out["config"]["dE_final_vs_oracle"] = scf["energy_final"] - float(f["energy_final"])
     if not args.no_extra:
-        out["extra"] = gradient_figures(mol, D, max(2, args.steps // 2), 1, not args.no_autotune)
+        ex = {}
+        steps2 = max(2, args.steps // 2)
+        ex.update(gradient_figures(mol, D, steps2, 1, not args.no_autotune))
         if not args.no_mixed:
-            out["extra"].update(mixed_figures(args.mixed_workload, max(2, args.steps // 2), 1))
+            ex.update(mixed_build_figures(mol, Hc, D, steps2, 1, not args.no_autotune))
+            if not args.no_scf:
+                g64 = H.rhf_gradient(mol, scf["D"], scf["F"])
+                ex.update(scf_figures(mol, scf, g64))
         if args.batch_molecules > 0:
-            out["extra"].update(batch_figures(args.batch_molecules, args.solo_sample))
+            ex.update(batch_figures(args.batch_molecules, args.solo_sample))
+        out["extra"] = ex
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(mol)
     print(json.dumps(out), flush=True)

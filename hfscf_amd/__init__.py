@@ -41,7 +41,7 @@ class ScfOptions(C.Structure):
                 ("diis_range", C.c_int), ("max_iter", C.c_int), ("schwarz", C.c_int), ("verbose", C.c_int),
                 ("energy_tol", C.c_double), ("rms_tol", C.c_double), ("integral_tol", C.c_double),
                 ("final_fp64", C.c_int), ("density_tol", C.c_double), ("graph", C.c_int), ("autotune", C.c_int),
-                ("reserved", C.c_int * 3)]
+                ("rebuild_every", C.c_int), ("device_step", C.c_int), ("polish_rms", C.c_double)]
 
 
 class ScfResult(C.Structure):
@@ -99,6 +99,8 @@ def hfgpu():
         lib.hfg_energy_backward.argtypes = [C.c_void_p, _dp, _dp]
         lib.hfg_eri_shell_quartet.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, _dp]
         lib.hfg_schwarz.argtypes = [C.c_void_p, _dp]
+        lib.hfg_copy_calibration.argtypes = [C.c_int, C.c_longlong, C.c_int, _dp]
+        lib.hfg_boys_probe.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp, C.c_int]
         lib.hfg_last_stats.argtypes = [C.c_void_p, C.POINTER(HfgStats)]
         lib.hfg_nbf.argtypes = [C.c_void_p]
         lib.hfg_last_error.argtypes = [C.c_void_p]
@@ -220,6 +222,17 @@ class Molecule:
 
 ANGSTROM_TO_BOHR = 1.8897261254535  # hfscf: src/molecule/hfscf_constants.hpp:5
 _SYMBOLS = {"H": 1, "HE": 2, "LI": 3, "BE": 4, "B": 5, "C": 6, "N": 7, "O": 8, "F": 9, "NE": 10}
+
+
+def copy_calibration(mib, reps=3, device=0):
+    """Runs `reps` launches of a plain copy kernel of `mib` MiB (reads and writes that many bytes
+    each) through libhfgpu; returns the milliseconds per copy. A kernel of known HBM traffic
+    for scaling a profiler's byte counters."""
+    ms = C.c_double(0.0)
+    rc = hfgpu().hfg_copy_calibration(int(device), int(mib * 2 ** 20), int(reps), C.byref(ms))
+    if rc:
+        raise RuntimeError(f"hfg_copy_calibration failed ({rc})")
+    return ms.value
 
 
 def read_xyz_frames(path):
@@ -458,6 +471,13 @@ class GpuContext:
         out = np.zeros(nf)
         self._check(hfgpu().hfg_eri_shell_quartet(self._h, s1, s2, s3, s4, _ptr(out)))
         return out
+
+    def boys(self, m, T, closed_form=False):
+        """F_0..F_m at the points T (len(T) x (m+1)) from the device Boys functions."""
+        T = np.ascontiguousarray(T, dtype=np.float64).reshape(-1)
+        F = np.zeros((len(T), m + 1))
+        self._check(hfgpu().hfg_boys_probe(self._h, int(m), len(T), _ptr(T), _ptr(F), int(closed_form)))
+        return F
 
     def schwarz(self):
         S = np.zeros((self.mol.ncart, self.mol.ncart))

@@ -632,6 +632,16 @@ int setup_geometry(hfg_ctx* c) {
     HFG_CHECK(c, c->d_gJ.upload(c->gJ, c->stream));
     HFG_CHECK(c, c->d_gK.upload(c->gK, c->stream));
     HFG_CHECK(c, c->d_shell_atom.upload(c->atom_local, c->stream));
+    {
+        std::vector<int> ao(c->nshell), nf(c->nshell);
+        for (int s = 0; s < c->nshell; ++s) {
+            ao[s] = c->pure ? c->pure_off[s] : c->cart_off[s];
+            nf[s] = nfun(c->L[s], c->pure);
+        }
+        HFG_CHECK(c, c->d_shell_ao.upload(ao, c->stream));
+        HFG_CHECK(c, c->d_shell_nf.upload(nf, c->stream));
+        HFG_CHECK(c, c->d_shell_mol.upload(c->shell_mol, c->stream));
+    }
     HFG_CHECK(c, c->d_mols.upload(md, c->stream));
     HFG_CHECK(c, c->d_mol_items.upload(mol_items, c->stream));
     HFG_CHECK(c, c->d_mol_items_off.upload(mol_items_off, c->stream));
@@ -682,10 +692,24 @@ int upload_dmax(hfg_ctx* c, const double* D) {
     return HFG_OK;
 }
 
-// The launches of one J/K build on `main` (class kernels fork to the side streams and join
-// back): counter reset, class kernels, gather into d_J / d_K. When ev_mid is given it is
-// recorded on main after the join (end of the class kernels).
-int enqueue_jk(hfg_ctx* c, hipStream_t main, bool dscreen, hipEvent_t ev_mid) {
+// Device buffers one J/K build reads and writes: the density, J and K, the work items' tile
+// slab and the class launches' queue counters (the context's own, or caller-provided ones).
+struct JkBuffers {
+    const double* D;
+    double* J;
+    double* K;
+    double* slab;
+    int* queue;
+};
+
+JkBuffers own_buffers(hfg_ctx* c) { return JkBuffers{c->d_D.p, c->d_J.p, c->d_K.p, c->d_slab.p, c->d_queue.p}; }
+
+// The class launches of the set `set` on `main` (forked to the class streams and joined back),
+// each preceded by the reset of its queue counter: the shared-stream classes' counters
+// [0, nshared) on the shared stream, the side classes' [nshared, ncls) on main before the fork.
+// With `timed`, every class launch is bracketed by events on its stream.
+int enqueue_classes(hfg_ctx* c, hipStream_t main, bool dscreen, bool timed, ClassSet set, const JkBuffers& buf,
+                    bool join_shared = true) {
     JkArgs a{};
     a.pairs = c->d_pairs.p;
     a.psmean = c->d_psmean.p;
@@ -696,59 +720,74 @@ int enqueue_jk(hfg_ctx* c, hipStream_t main, bool dscreen, hipEvent_t ev_mid) {
     a.pp1 = c->d_pp1.p;
     a.pp2 = c->d_pp2.p;
     a.boys = c->d_boys.p;
-    a.D = c->d_D.p;
+    a.D = buf.D;
     a.mols = c->d_mols.p;
     a.tol = c->tol;
     a.psq = c->d_psq.p;
     a.elem_tol = c->tol;
     a.prim_cut = c->opt.integral_tol / 1e4;
-    a.slab = c->d_slab.p;
+    a.slab = buf.slab;
     a.dmax = c->d_dmax.p;
     a.dtol = dscreen ? c->opt.density_tol : 0.0;
     const int mixed = c->opt.precision == HFG_MIXED ? 1 : 0;
-    HFG_CHECK(c, launch_zero_ints(c->d_queue.p, (int)c->classes.size(), main));
-    size_t k = 0;
-    const bool timed = ev_mid != nullptr;  // direct launches: bracket every class launch with events
-    HFG_CHECK(c, for_each_class_concurrent(c, main, [&](ClassWork& cw, hipStream_t st) {
+    const int ncls = (int)c->classes.size(), nshared = shared_class_count(c, false);
+    if (set != ClassSet::Shared) HFG_CHECK(c, launch_zero_ints(buf.queue + nshared, ncls - nshared, main));
+    bool shared_zeroed = false;
+    HFG_CHECK(c, for_each_class_concurrent(c, main, [&](ClassWork& cw, hipStream_t st, int qi) {
+        hipError_t e = hipSuccess;
+        if (st == c->heavy && !shared_zeroed) {
+            if ((e = launch_zero_ints(buf.queue, nshared, st)) != hipSuccess) return e;
+            shared_zeroed = true;
+        }
         JkArgs b = a;
         b.items = c->d_items.p + cw.dev_off;
-        b.queue = c->d_queue.p + (k++);
+        b.queue = buf.queue + qi;
         if (timed) {
-            hipError_t e = hipSuccess;
             if (!cw.ev_start && (e = hipEventCreate(&cw.ev_start)) != hipSuccess) return e;
             if (!cw.ev_stop && (e = hipEventCreate(&cw.ev_stop)) != hipSuccess) return e;
             if ((e = hipEventRecord(cw.ev_start, st)) != hipSuccess) return e;
         }
-        hipError_t e = launch_jk(cw.la, cw.lb, cw.lc, cw.ld, c->pure, mixed, b, (int)cw.items.size(), cw.jk_shape, st);
+        e = launch_jk(cw.la, cw.lb, cw.lc, cw.ld, c->pure, mixed, b, (int)cw.items.size(), cw.jk_shape, st);
         if (e == hipSuccess && timed) e = hipEventRecord(cw.ev_stop, st);
         return e;
-    }));
+    }, false, set, join_shared));
     c->class_timed = timed;
-    if (ev_mid) HFG_CHECK(c, hipEventRecord(ev_mid, main));
+    return HFG_OK;
+}
+
+// The gather of every work item's tiles into J and K, on `main`.
+int enqueue_gather(hfg_ctx* c, hipStream_t main, const JkBuffers& buf) {
     GatherArgs g{};
     g.tiles = c->d_tiles.p;
     g.gJ_off = c->d_gJ_off.p;
     g.gJ = c->d_gJ.p;
     g.gK_off = c->d_gK_off.p;
     g.gK = c->d_gK.p;
-    g.slab = c->d_slab.p;
+    g.slab = buf.slab;
     g.blk_nf = c->d_blk_nf.p;
     g.blk_func_off = c->d_blk_func_off.p;
     g.blk_func_ao = c->d_blk_func_ao.p;
     g.mols = c->d_mols.p;
-    g.J = c->d_J.p;
-    g.K = c->d_K.p;
+    g.J = buf.J;
+    g.K = buf.K;
     HFG_CHECK(c, launch_gather(g, (int)c->tiles.size(), main));
     return HFG_OK;
 }
 
 // J and K of the densities already in d_D, into d_J / d_K, directly or by replaying the
 // captured graph of this configuration. ev0 / ev1 bracket the build (the class kernels only,
-// when launched directly).
+// when launched directly). A graph holds only the side-stream classes: the classes of the
+// device's shared stream are launched directly on it at every replay, so no capture ever
+// records work of that stream (other contexts may enqueue there concurrently) and its kernels
+// keep running on the one queue that holds their scratch.
 int run_jk(hfg_ctx* c, bool dscreen = false) {
+    const JkBuffers buf = own_buffers(c);
     if (!c->opt.graph) {
         HFG_CHECK(c, hipEventRecord(c->ev0, c->stream));
-        return enqueue_jk(c, c->stream, dscreen, c->ev1);
+        int r = enqueue_classes(c, c->stream, dscreen, true, ClassSet::All, buf);
+        if (r) return r;
+        HFG_CHECK(c, hipEventRecord(c->ev1, c->stream));
+        return enqueue_gather(c, c->stream, buf);
     }
     const int mixed = c->opt.precision == HFG_MIXED ? 1 : 0;
     JkGraph* gr = nullptr;
@@ -757,7 +796,7 @@ int run_jk(hfg_ctx* c, bool dscreen = false) {
     if (!gr) {
         hipGraph_t graph = nullptr;
         HFG_CHECK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
-        const int r = enqueue_jk(c, c->stream, dscreen, nullptr);
+        const int r = enqueue_classes(c, c->stream, dscreen, false, ClassSet::Side, buf);
         const hipError_t e = hipStreamEndCapture(c->stream, &graph);
         if (r != HFG_OK || e != hipSuccess) {
             if (graph) (void)hipGraphDestroy(graph);
@@ -774,9 +813,15 @@ int run_jk(hfg_ctx* c, bool dscreen = false) {
         gr = &c->graphs.back();
     }
     HFG_CHECK(c, hipEventRecord(c->ev0, c->stream));
+    const bool shared = shared_class_count(c, false) > 0;
+    if (shared) {  // concurrent with the graph: main waits for the shared stream after the launch
+        const int r = enqueue_classes(c, c->stream, dscreen, false, ClassSet::Shared, buf, false);
+        if (r) return r;
+    }
     HFG_CHECK(c, hipGraphLaunch(gr->exec, c->stream));
+    if (shared) HFG_CHECK(c, hipStreamWaitEvent(c->stream, c->ev_heavy_join, 0));
     HFG_CHECK(c, hipEventRecord(c->ev1, c->stream));
-    return HFG_OK;
+    return enqueue_gather(c, c->stream, buf);
 }
 
 int finish_stats(hfg_ctx* c, std::chrono::steady_clock::time_point t0) {
@@ -1026,6 +1071,12 @@ int hfg_fock(hfg_ctx* c, const double* H, const double* D, int incremental, doub
     auto t0 = std::chrono::steady_clock::now();
     const size_t nn = (size_t)c->nn_total;
     const bool inc = incremental && c->have_prev;
+    if (inc && c->prev_on_device) {
+        c->Dprev.resize(nn);
+        HFG_CHECK(c, hipMemcpyAsync(c->Dprev.data(), c->d_Dprev.p, nn * sizeof(double), hipMemcpyDeviceToHost,
+                                    c->stream));
+        HFG_CHECK(c, hipStreamSynchronize(c->stream));
+    }
     std::vector<double> Dd;
     const double* Duse = D;
     if (inc) {
@@ -1042,6 +1093,34 @@ int hfg_fock(hfg_ctx* c, const double* H, const double* D, int incremental, doub
     HFG_CHECK(c, hipMemcpyAsync(F, c->d_F.p, nn * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     c->Dprev.assign(D, D + nn);
     c->have_prev = true;
+    c->prev_on_device = false;
+    return finish_stats(c, t0);
+}
+
+int hfg_fock_device(hfg_ctx* c, const double* H, const double* D, int incremental, double* F) {
+    int r = check_ready(c);
+    if (r) return r;
+    if (!H || !D || !F) return fail(c, HFG_ERR_ARG, "hfg_fock_device: NULL argument");
+    auto t0 = std::chrono::steady_clock::now();
+    const size_t nn = (size_t)c->nn_total;
+    const bool inc = incremental && c->have_prev;
+    HFG_CHECK(c, c->d_Dprev.ensure(nn));
+    if (inc && !c->prev_on_device)
+        HFG_CHECK(c, hipMemcpyAsync(c->d_Dprev.p, c->Dprev.data(), nn * sizeof(double), hipMemcpyHostToDevice,
+                                    c->stream));
+    if (inc)
+        HFG_CHECK(c, launch_axpby(1.0, D, -1.0, c->d_Dprev.p, c->d_D.p, nn, c->stream));  // D - D_prev
+    else
+        HFG_CHECK(c, hipMemcpyAsync(c->d_D.p, D, nn * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    const bool dscreen = inc && c->opt.density_tol > 0.0;
+    if (dscreen)
+        HFG_CHECK(c, launch_block_absmax(c->d_D.p, c->d_mols.p, c->d_shell_ao.p, c->d_shell_nf.p, c->d_shell_mol.p,
+                                         c->nshell, c->d_dmax.p, c->stream));
+    if ((r = run_jk(c, dscreen))) return r;
+    HFG_CHECK(c, launch_fock(H, c->d_J.p, c->d_K.p, F, c->d_G.p, (int)nn, inc ? 1 : 0, c->stream));
+    HFG_CHECK(c, hipMemcpyAsync(c->d_Dprev.p, D, nn * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    c->have_prev = true;
+    c->prev_on_device = true;
     return finish_stats(c, t0);
 }
 
@@ -1182,6 +1261,62 @@ int hfg_eri_shell_quartet(hfg_ctx* c, int s1, int s2, int s3, int s4, double* bl
     return HFG_OK;
 }
 
+int hfg_boys_probe(hfg_ctx* c, int m, int n, const double* T, double* F, int closed_form) {
+    if (!c) return HFG_ERR_ARG;
+    if (!T || !F || n < 0 || m < 0 || m > kBoysMaxM || (closed_form && m > 1))
+        return fail(c, HFG_ERR_ARG, "hfg_boys_probe: bad arguments");
+    for (int i = 0; i < n; ++i)
+        if (!(T[i] >= 0.0)) return fail(c, HFG_ERR_ARG, "hfg_boys_probe: T must be >= 0");
+    if (n == 0) return HFG_OK;
+    DevBuf<double> dT, dF;
+    struct Release {  // DevBuf does not free itself
+        DevBuf<double>& a;
+        DevBuf<double>& b;
+        ~Release() {
+            a.release();
+            b.release();
+        }
+    } release{dT, dF};
+    HFG_CHECK(c, dT.ensure((size_t)n));
+    HFG_CHECK(c, dF.ensure((size_t)n * (m + 1)));
+    HFG_CHECK(c, hipMemcpyAsync(dT.p, T, (size_t)n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HFG_CHECK(c, launch_boys_probe(m, dT.p, c->d_boys.p, dF.p, n, closed_form, c->stream));
+    HFG_CHECK(c, hipMemcpyAsync(F, dF.p, (size_t)n * (m + 1) * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HFG_CHECK(c, hipStreamSynchronize(c->stream));
+    return HFG_OK;
+}
+
+int hfg_copy_calibration(int device, long long bytes, int reps, double* ms_per_copy) {
+    if (bytes < 16 || reps < 1) return HFG_ERR_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return HFG_ERR_UNSUPPORTED;
+    if (hipSetDevice(device) != hipSuccess) return HFG_ERR_HIP;
+    const size_t n = (size_t)bytes / sizeof(double2);
+    double2 *a = nullptr, *b = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = HFG_OK;
+    if (hipMalloc((void**)&a, n * sizeof(double2)) != hipSuccess || hipMalloc((void**)&b, n * sizeof(double2)) != hipSuccess)
+        rc = HFG_ERR_NOMEM;
+    if (rc == HFG_OK && (hipStreamCreate(&st) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+                         hipEventCreate(&e1) != hipSuccess || hipMemsetAsync(a, 0, n * sizeof(double2), st) != hipSuccess))
+        rc = HFG_ERR_HIP;
+    if (rc == HFG_OK && launch_copy(a, b, n, st) != hipSuccess) rc = HFG_ERR_HIP;  // warm-up
+    if (rc == HFG_OK && hipEventRecord(e0, st) != hipSuccess) rc = HFG_ERR_HIP;
+    for (int k = 0; rc == HFG_OK && k < reps; ++k)
+        if (launch_copy(a, b, n, st) != hipSuccess) rc = HFG_ERR_HIP;
+    if (rc == HFG_OK && (hipEventRecord(e1, st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess)) rc = HFG_ERR_HIP;
+    float ms = 0.0f;
+    if (rc == HFG_OK && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = HFG_ERR_HIP;
+    if (rc == HFG_OK && ms_per_copy) *ms_per_copy = ms / reps;
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    return rc;
+}
+
 int hfg_schwarz(hfg_ctx* c, double* screen) {
     int r = check_ready(c);
     if (r) return r;
@@ -1237,6 +1372,10 @@ void hfg_destroy(hfg_ctx* c) {
     c->d_gJ_off.release();
     c->d_gK_off.release();
     c->d_shell_atom.release();
+    c->d_shell_ao.release();
+    c->d_shell_nf.release();
+    c->d_shell_mol.release();
+    c->d_Dprev.release();
     c->d_queue.release();
     c->d_mol_items.release();
     c->d_mol_items_off.release();

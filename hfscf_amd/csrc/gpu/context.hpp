@@ -1,6 +1,7 @@
 // Context of the hfgpu library (host state and device buffers), shared by the
 // J/K driver (context.hip) and the gradient driver (grad.hip).
 #pragma once
+#include <algorithm>
 #include <hip/hip_runtime.h>
 
 #include <string>
@@ -147,9 +148,13 @@ struct hfg_ctx {
     // launch graphs of the J/K build (keyed by precision and density screening)
     std::vector<hfg::JkGraph> graphs;
     bool class_timed = false;  // the last J/K build bracketed each class launch with events
-    // incremental Fock state
-    bool have_prev = false;
+    // incremental Fock state: the previous density, on the host (hfg_fock) or on the device
+    // (hfg_fock_device, prev_on_device)
+    bool have_prev = false, prev_on_device = false;
     std::vector<double> Dprev;
+    DevBuf<double> d_Dprev;
+    // per shell (global index): output-basis offset within its molecule, function count, molecule
+    DevBuf<int> d_shell_ao, d_shell_nf, d_shell_mol;
     hfg_stats stats{};
 };
 
@@ -173,26 +178,51 @@ namespace hfg {
 // Launch fn(class_work, stream) for every class on the side streams (fork from and join back
 // to `main`): classes are independent until the gather. Under stream capture the fork/join
 // events make the side-stream launches part of the captured graph.
-template <typename F>
-inline hipError_t for_each_class_concurrent(hfg_ctx* c, hipStream_t main, F&& fn, bool grad = false) {
+// Which class launches a call enqueues: every class, only those on the side streams (the part of
+// a J/K build that is captured in a graph), or only those on the shared stream.
+enum class ClassSet { All, Side, Shared };
+
+// Number of classes that run on the shared stream (J/K, or the gradient when grad). Their
+// device-queue counters come first ([0, n)), the side-stream classes' after them.
+inline int shared_class_count(const hfg_ctx* c, bool grad) {
     const std::vector<int>& streams = grad ? c->grad_stream : c->class_stream;
+    return (int)std::count(streams.begin(), streams.end(), -1);
+}
+
+// Forks `main` to the streams of the selected classes, calls fn(class, stream, counter index)
+// for each, and joins them back into `main` (the shared stream only with join_shared). Counter indices: shared-stream classes 0.., then the
+// side-stream classes, in launch order.
+template <typename F>
+inline hipError_t for_each_class_concurrent(hfg_ctx* c, hipStream_t main, F&& fn, bool grad = false,
+                                            ClassSet set = ClassSet::All, bool join_shared = true) {
+    const std::vector<int>& streams = grad ? c->grad_stream : c->class_stream;
+    const bool side = set != ClassSet::Shared, shared = set != ClassSet::Side;
     hipError_t e = hipEventRecord(c->ev_fork, main);
     if (e != hipSuccess) return e;
-    for (int k = 0; k < kSideStreams; ++k)
-        if ((e = hipStreamWaitEvent(c->side[k], c->ev_fork, 0)) != hipSuccess) return e;
-    if ((e = hipStreamWaitEvent(c->heavy, c->ev_fork, 0)) != hipSuccess) return e;
+    if (side)
+        for (int k = 0; k < kSideStreams; ++k)
+            if ((e = hipStreamWaitEvent(c->side[k], c->ev_fork, 0)) != hipSuccess) return e;
+    if (shared && (e = hipStreamWaitEvent(c->heavy, c->ev_fork, 0)) != hipSuccess) return e;
     // the shared-stream classes first: they are few workgroups of long latency, and launched
     // after the side streams' persistent grids they would wait for compute units to drain
+    int q = 0;
     for (int pass = 0; pass < 2; ++pass)
         for (size_t i = 0; i < c->classes.size(); ++i)
-            if ((streams[i] < 0) == (pass == 0))
-                if ((e = fn(c->classes[i], streams[i] < 0 ? c->heavy : c->side[streams[i]])) != hipSuccess) return e;
-    for (int k = 0; k < kSideStreams; ++k) {
-        if ((e = hipEventRecord(c->ev_join[k], c->side[k])) != hipSuccess) return e;
-        if ((e = hipStreamWaitEvent(main, c->ev_join[k], 0)) != hipSuccess) return e;
-    }
+            if ((streams[i] < 0) == (pass == 0)) {
+                const int qi = q++;
+                if ((pass == 0 && !shared) || (pass == 1 && !side)) continue;
+                if ((e = fn(c->classes[i], streams[i] < 0 ? c->heavy : c->side[streams[i]], qi)) != hipSuccess)
+                    return e;
+            }
+    if (side)
+        for (int k = 0; k < kSideStreams; ++k) {
+            if ((e = hipEventRecord(c->ev_join[k], c->side[k])) != hipSuccess) return e;
+            if ((e = hipStreamWaitEvent(main, c->ev_join[k], 0)) != hipSuccess) return e;
+        }
+    if (!shared) return hipSuccess;
     if ((e = hipEventRecord(c->ev_heavy_join, c->heavy)) != hipSuccess) return e;
-    return hipStreamWaitEvent(main, c->ev_heavy_join, 0);
+    // without join_shared the caller makes main wait for ev_heavy_join itself, later
+    return join_shared ? hipStreamWaitEvent(main, c->ev_heavy_join, 0) : hipSuccess;
 }
 
 // Stream of a class launch (J/K or gradient).

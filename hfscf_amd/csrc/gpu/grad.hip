@@ -77,11 +77,10 @@ int run_gradient(hfg_ctx* c, const double* D, double* grad) {
     HFG_CHECK(c, ensure_grad_ws(c, mixed));
     HFG_CHECK(c, hipEventRecord(c->ev0, c->stream));
     HFG_CHECK(c, launch_zero_ints(c->d_queue.p + ncls, ncls, c->stream));
-    size_t k = 0;
-    HFG_CHECK(c, for_each_class_concurrent(c, c->stream, [&](const ClassWork& cw, hipStream_t st) {
+    HFG_CHECK(c, for_each_class_concurrent(c, c->stream, [&](const ClassWork& cw, hipStream_t st, int qi) {
         GradArgs b = a;
         b.items = c->d_items.p + cw.dev_off;
-        b.queue = c->d_queue.p + ncls + (k++);
+        b.queue = c->d_queue.p + ncls + qi;
         return launch_grad(cw.la, cw.lb, cw.lc, cw.ld, c->pure, mixed, b, c->d_gws.p, (int)cw.items.size(),
                            cw.grad_shape, st);
     }, true));

@@ -104,6 +104,102 @@ hipError_t launch_zero_ints(int* p, int n, hipStream_t s) {
     return hipGetLastError();
 }
 
+// Plain streaming copy, 16 bytes per lane and iteration (a kernel of known traffic, to scale a
+// profiler's HBM byte counters).
+__global__ __launch_bounds__(256) void copy_kernel(const double2* __restrict__ src, double2* __restrict__ dst,
+                                                   size_t n) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
+hipError_t launch_copy(const double2* src, double2* dst, size_t n, hipStream_t s) {
+    hipLaunchKernelGGL(copy_kernel, dim3(4096), dim3(256), 0, s, src, dst, n);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ device-side density helpers
+// z = a x + b y (the density difference of an incremental build made on the device).
+__global__ __launch_bounds__(256) void axpby_kernel(double a, const double* __restrict__ x, double b,
+                                                    const double* __restrict__ y, double* __restrict__ z, size_t n) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) z[i] = a * x[i] + b * y[i];
+}
+
+hipError_t launch_axpby(double a, const double* x, double b, const double* y, double* z, size_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    const unsigned g = (unsigned)std::min<size_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(axpby_kernel, dim3(g), dim3(256), 0, s, a, x, b, y, z, n);
+    return hipGetLastError();
+}
+
+// Largest |D| of every shell block of every molecule (one workgroup per shell row), rounded up
+// to the next float so the single-precision table never understates the density-screening
+// bound; the device form of the table the host builds for hfg_fock.
+__global__ __launch_bounds__(64) void block_absmax_kernel(const double* __restrict__ D, const MolDev* __restrict__ mols,
+                                                          const int* __restrict__ shell_ao,
+                                                          const int* __restrict__ shell_nf,
+                                                          const int* __restrict__ shell_mol, float* __restrict__ dmax) {
+    const int ga = blockIdx.x;
+    const MolDev md = mols[shell_mol[ga]];
+    const int a = ga - md.shell0, oa = shell_ao[ga], na = shell_nf[ga], n = md.n;
+    const double* Dm = D + md.moff;
+    for (int b = threadIdx.x; b < md.nshell; b += blockDim.x) {
+        const int gb = md.shell0 + b, ob = shell_ao[gb], nb = shell_nf[gb];
+        double mx = 0.0;
+        for (int p = 0; p < na; ++p)
+            for (int q = 0; q < nb; ++q) mx = fmax(mx, fabs(Dm[(size_t)(oa + p) * n + ob + q]));
+        dmax[md.dm_off + a * md.nshell + b] = mx > 0.0 ? nextafterf((float)mx, INFINITY) : 0.0f;
+    }
+}
+
+hipError_t launch_block_absmax(const double* D, const MolDev* mols, const int* shell_ao, const int* shell_nf,
+                               const int* shell_mol, int nshell, float* dmax, hipStream_t s) {
+    if (nshell <= 0) return hipSuccess;
+    hipLaunchKernelGGL(block_absmax_kernel, dim3(nshell), dim3(64), 0, s, D, mols, shell_ao, shell_nf, shell_mol, dmax);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ Boys probe (test hook)
+// F_0..F_M at n values of T through the device functions the class kernels use: the table
+// with Taylor step and recursions (boys_eval_rs), or the closed forms of F_0 and F_1 (boys_low).
+template <int M>
+__global__ void boys_probe_kernel(const double* __restrict__ T, const double* __restrict__ tab, double* out, int n,
+                                  int closed_form) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double F[M + 1];
+    if constexpr (M <= 1) {
+        if (closed_form) {
+            boys_low<M, double>(T[i], 1.0, F);
+        } else {
+            boys_eval_rs<M, double>(T[i], tab, 1.0, F);
+        }
+    } else {
+        boys_eval_rs<M, double>(T[i], tab, 1.0, F);
+    }
+    for (int m = 0; m <= M; ++m) out[(size_t)i * (M + 1) + m] = F[m];
+}
+
+template <int M>
+hipError_t launch_boys_probe_m(int m, const double* T, const double* tab, double* out, int n, int closed_form,
+                               hipStream_t s) {
+    if constexpr (M > kBoysMaxM) {
+        return hipErrorInvalidValue;
+    } else {
+        if (m != M) return launch_boys_probe_m<M + 1>(m, T, tab, out, n, closed_form, s);
+        hipLaunchKernelGGL(boys_probe_kernel<M>, dim3((n + 127) / 128), dim3(128), 0, s, T, tab, out, n,
+                           closed_form);
+        return hipGetLastError();
+    }
+}
+
+hipError_t launch_boys_probe(int m, const double* T, const double* tab, double* out, int n, int closed_form,
+                             hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    if (m < 0 || m > kBoysMaxM || (closed_form && m > 1)) return hipErrorInvalidValue;
+    return launch_boys_probe_m<0>(m, T, tab, out, n, closed_form, s);
+}
+
 // ------------------------------------------------------------------ Fock
 // G = 2J - K (+ Facc when accumulating); F = base + G; Facc keeps G for incremental builds.
 __global__ void fock_kernel(const double* base, const double* J, const double* K, double* F, double* Facc, int nn,

@@ -61,7 +61,7 @@ hfg_options gpu_options(const hfscf_scf_options& o) {
 // GPU backend: ERIOS/hf_solver/scf_gradient two-electron work through the hfgpu C ABI.
 class GpuJK final : public JKBuilder {
    public:
-    GpuJK(const Molecule& mol, const hfscf_scf_options& o) : mol_(mol), precision_(o.precision) {
+    GpuJK(const Molecule& mol, const hfscf_scf_options& o) : mol_(mol), precision_(o.precision), device_(o.device) {
         const hfg_options go = gpu_options(o);
         int r = hfg_create(&ctx_, o.device, &go);
         if (r) throw std::runtime_error("hfg_create failed (" + std::to_string(r) + "): no usable GPU");
@@ -76,9 +76,22 @@ class GpuJK final : public JKBuilder {
         // mixed precision: FP32 only for builds on a density difference, whose FP32 error
         // shrinks with the difference; full builds (the first, the final one) in FP64
         const bool inc = incremental && have_prev_;
-        check(hfg_set_precision(ctx_, (precision_ == HFG_MIXED && inc && !final_) ? HFG_MIXED : HFG_FP64));
+        check(hfg_set_precision(ctx_, (precision_ == HFG_MIXED && inc && !final_ && !fp64_) ? HFG_MIXED : HFG_FP64));
         check(hfg_fock(ctx_, H, D, incremental, F));
         have_prev_ = true;
+    }
+    bool fock_device(const double* H, const double* D, int incremental, double* F) override {
+        const bool inc = incremental && have_prev_;
+        check(hfg_set_precision(ctx_, (precision_ == HFG_MIXED && inc && !final_ && !fp64_) ? HFG_MIXED : HFG_FP64));
+        check(hfg_fock_device(ctx_, H, D, incremental, F));
+        have_prev_ = true;
+        return true;
+    }
+    int device() const override { return device_; }
+    void begin_fp64_phase() override {
+        fp64_ = true;
+        have_prev_ = false;
+        check(hfg_reset_incremental(ctx_));
     }
     void gradient_2e(const double* D, double* grad) override {
         std::vector<double> g((size_t)mol_.natom() * 3);
@@ -95,7 +108,8 @@ class GpuJK final : public JKBuilder {
     const Molecule& mol_;
     hfg_ctx* ctx_ = nullptr;
     int precision_;
-    bool have_prev_ = false, final_ = false;
+    int device_;
+    bool have_prev_ = false, final_ = false, fp64_ = false;
 };
 
 // Callback backend (an external Fock builder behind the same seam).
@@ -168,6 +182,7 @@ int batch_rhf(int nmol, hfscf_mol* const* mols, const hfscf_scf_options& opt, hf
     so.incremental = opt.incremental != 0;
     so.final_build = opt.final_fp64 != 0;
     so.verbose = 0;
+    so.rebuild_every = opt.rebuild_every;  // no FP64 polish phase in batches (so.mixed stays false)
     std::vector<std::unique_ptr<hf_solver>> sol(nmol);
     std::vector<double> H(nn), D(nn), F(nn);
     // one-electron set-up per molecule (independent, so in parallel)
@@ -187,6 +202,7 @@ int batch_rhf(int nmol, hfscf_mol* const* mols, const hfscf_scf_options& opt, hf
     if (!err.empty()) throw std::runtime_error(err);
     S.setup_seconds = secs(t_setup, clk::now());
     std::vector<char> active(nmol, 1), conv(nmol, 0);
+    int chain_start = 1;  // iteration of the last full build
     const auto t_scf = clk::now();
     for (int it = 1;; ++it) {
         int nact = 0;
@@ -194,8 +210,12 @@ int batch_rhf(int nmol, hfscf_mol* const* mols, const hfscf_scf_options& opt, hf
         if (!nact) break;
         for (int m = 0; m < nmol; ++m)
             std::copy(sol[m]->density().begin(), sol[m]->density().end(), D.begin() + moff[m]);
-        // as GpuJK: the first (full) build in FP64, density-difference builds in the chosen precision
-        const int inc = (opt.incremental && it > 1) ? 1 : 0;
+        // as GpuJK: the first (full) build in FP64, density-difference builds in the chosen
+        // precision; every rebuild_every-th build is a full one (as hf_solver::wants_incremental
+        // decides for a solo run, so that each molecule follows its solo trajectory)
+        const int inc = (opt.incremental && it > chain_start &&
+                         (opt.rebuild_every <= 0 || (it - chain_start) % opt.rebuild_every != 0)) ? 1 : 0;
+        if (!inc) chain_start = it;
         check(hfg_set_precision(ctx, (opt.precision == HFG_MIXED && inc) ? HFG_MIXED : HFG_FP64));
         const auto t0 = clk::now();
         check(hfg_fock(ctx, H.data(), D.data(), inc, F.data()));
@@ -312,6 +332,9 @@ void hfscf_default_scf_options(hfscf_scf_options* o) {
     o->integral_tol = 1e-14;
     o->final_fp64 = 1;
     o->density_tol = 1e-11;
+    o->rebuild_every = 8;
+    o->device_step = 1;
+    o->polish_rms = 1e-6;
 }
 
 hfscf_mol* hfscf_mol_create(int natom, const int* Z, const double* xyz_bohr, const char* gbs_path, int pure_override) {
@@ -441,6 +464,10 @@ int hfscf_rhf_run(hfscf_mol* m, const hfscf_scf_options* o, const hfscf_callback
         so.incremental = opt.incremental != 0;
         so.final_build = opt.final_fp64 != 0;
         so.verbose = opt.verbose;
+        so.rebuild_every = opt.rebuild_every;
+        so.polish_rms = opt.polish_rms;
+        so.mixed = opt.precision == HFG_MIXED;
+        so.device_step = opt.device_step != 0;
         hf_solver solver(m->mol, so, jk.get());
         solver.init_data();
         bool conv = solver.scf_run();

@@ -97,6 +97,7 @@ void hf_solver::init_data() {
     nuclear_attraction(mol_, V_);
     H_.resize(nn);
     for (size_t k = 0; k < nn; ++k) H_[k] = T_[k] + V_[k];
+    if (on_device()) return;  // X, the guess and every later step are formed on the GPU
     // symmetric orthogonalisation X = S^-1/2 (hfscf_hf_solver.cpp:184-204)
     std::vector<double> w(n_), U(nn);
     eigh(n_, S_.data(), w.data(), U.data());
@@ -142,10 +143,28 @@ void hf_solver::calc_scf_energy() {
     energies_.push_back(e);
 }
 
+bool hf_solver::wants_incremental() const {
+    const int next = iteration_ + 1;
+    if (!opt_.incremental || next <= chain_start_) return false;
+    return opt_.rebuild_every <= 0 || (next - chain_start_) % opt_.rebuild_every != 0;
+}
+
+bool hf_solver::wants_polish() const {
+    return opt_.mixed && opt_.polish_rms > 0.0 && !polish_ && diis_->current_rms() < opt_.polish_rms;
+}
+
+void hf_solver::begin_polish() {
+    polish_ = true;
+    chain_start_ = iteration_ + 1;  // the next build is a full FP64 build
+    diis_ = std::make_unique<diis_solver>(opt_.diis_range);
+}
+
 void hf_solver::update_fock_matrix() {
+    const bool inc = wants_incremental();
     ++iteration_;
+    if (!inc) chain_start_ = iteration_;
     auto t0 = std::chrono::steady_clock::now();
-    jk_->fock(H_.data(), D_.data(), (opt_.incremental && iteration_ > 1) ? 1 : 0, F_.data());
+    jk_->fock(H_.data(), D_.data(), inc ? 1 : 0, F_.data());
     auto t1 = std::chrono::steady_clock::now();
     fock_seconds_ = std::chrono::duration<double>(t1 - t0).count();
     fock_total_ += fock_seconds_;
@@ -163,6 +182,7 @@ void hf_solver::after_fock() {
 void hf_solver::begin() { diis_ = std::make_unique<diis_solver>(opt_.diis_range); }
 
 void hf_solver::step(const double* F, double fock_seconds) {
+    if (!wants_incremental()) chain_start_ = iteration_ + 1;
     ++iteration_;
     std::copy(F, F + (size_t)n_ * n_, F_.begin());
     fock_seconds_ = fock_seconds;
@@ -171,7 +191,8 @@ void hf_solver::step(const double* F, double fock_seconds) {
 }
 
 bool hf_solver::converged() const {
-    // hfscf_hf_solver.cpp:415
+    // hfscf_hf_solver.cpp:415; a mixed run with polishing converges only in its FP64 phase
+    if (opt_.mixed && opt_.polish_rms > 0.0 && !polish_) return false;
     return diis_->current_rms() < opt_.rms_tol && std::fabs(e_cur_ - e_prev_) < opt_.energy_tol;
 }
 
@@ -185,6 +206,7 @@ void hf_solver::finish(double e_final_elec) {
 }
 
 bool hf_solver::scf_run() {
+    if (on_device()) return scf_run_device();
     begin();
     bool conv = false;
     for (;;) {
@@ -192,6 +214,10 @@ bool hf_solver::scf_run() {
         if (opt_.verbose)
             std::fprintf(stderr, "  iter %3d  E = %.12f  dE = %.3e  rms = %.3e  fock %.4fs\n", iteration_, e_cur_,
                          e_cur_ - e_prev_, diis_->current_rms(), fock_seconds_);
+        if (wants_polish()) {
+            begin_polish();
+            jk_->begin_fp64_phase();
+        }
         if (converged()) {
             conv = true;
             break;

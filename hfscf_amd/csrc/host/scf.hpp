@@ -26,6 +26,16 @@ struct JKBuilder {
     virtual void gradient_2e(const double* D, double* grad) = 0;
     // Full-precision rebuild hint for the final iteration of a mixed-precision run.
     virtual void set_final(bool final_build) { (void)final_build; }
+    // End of the reduced-precision phase of a mixed-precision run: every later build is FP64,
+    // and the next one a full build (the accumulated FP32 error of the incremental chain goes).
+    virtual void begin_fp64_phase() {}
+    // Device-resident form of fock(): H, D and F are device pointers on device(). A backend
+    // without one returns false (device() < 0), and the solver keeps its host loop.
+    virtual bool fock_device(const double* H, const double* D, int incremental, double* F) {
+        (void)H, (void)D, (void)incremental, (void)F;
+        return false;
+    }
+    virtual int device() const { return -1; }
 };
 
 struct ScfOptions {
@@ -37,6 +47,17 @@ struct ScfOptions {
     bool incremental = false;  // incremental (delta-D) Fock builds
     bool final_build = true;   // full-precision Fock build at the converged density
     int verbose = 0;
+    // incremental builds: every rebuild_every-th iteration is a full build, which removes the
+    // error that density screening accumulates along the incremental chain (0 = never)
+    int rebuild_every = 0;
+    // mixed precision: once the DIIS rms falls below polish_rms, the run continues with FP64
+    // builds from a fresh full build and a fresh DIIS subspace, and converges to rms_tol /
+    // energy_tol there, so D and F are those of the FP64 SCF (0 = stay mixed to the end)
+    double polish_rms = 0.0;
+    bool mixed = false;
+    // with a backend that builds on device memory, run the whole SCF step on the GPU
+    // (scf_run_device: DIIS, diagonalisation, density and energy in rocBLAS / rocSOLVER)
+    bool device_step = true;
 };
 
 class diis_solver {
@@ -69,7 +90,14 @@ class hf_solver {
     void step(const double* F, double fock_seconds);
     bool converged() const;
     void finish(double e_final_elec);
-    bool wants_incremental() const { return opt_.incremental && iteration_ >= 1; }
+    // incremental build for the next iteration (iteration_ + 1): after the first, except every
+    // rebuild_every-th and the first after the switch to FP64
+    bool wants_incremental() const;
+    // mixed precision: the switch to FP64 is due (DIIS rms below polish_rms); begin_polish()
+    // restarts DIIS and counts the iterations of the FP64 phase
+    bool wants_polish() const;
+    void begin_polish();
+    bool polishing() const { return polish_; }
     const ScfOptions& options() const { return opt_; }
 
     double energy_electronic() const { return e_cur_; }
@@ -93,6 +121,8 @@ class hf_solver {
     void after_fock();                  // DIIS, F' = X^T F X, density and energy
     void calc_density_matrix();         // hfscf: src/scfsolver/hfscf_hf_solver.cpp:206-242
     void calc_scf_energy();             // hfscf: src/scfsolver/hfscf_hf_solver.cpp:244-252
+    bool on_device() const { return opt_.device_step && jk_ && jk_->device() >= 0; }
+    bool scf_run_device();  // the loop of scf_run with every matrix on the GPU (devscf.cpp)
 
     const Molecule& mol_;
     ScfOptions opt_;
@@ -102,6 +132,9 @@ class hf_solver {
     double e_cur_ = 0.0, e_prev_ = 0.0, e_final_ = 0.0;
     double fock_seconds_ = 0.0, fock_total_ = 0.0;
     std::unique_ptr<diis_solver> diis_;
+    bool polish_ = false;
+    int chain_start_ = 1;  // iteration of the last full build
+    double device_rms_ = 0.0;
 };
 
 // RHF analytic gradient (natom x 3) from the converged half density D and Fock matrix F:

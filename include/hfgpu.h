@@ -84,6 +84,13 @@ int hfg_build_jk(hfg_ctx* ctx, const double* D, double* J, double* K, int flags)
  * digestion (hfscf_hf_solver.cpp:261-278). */
 int hfg_fock(hfg_ctx* ctx, const double* H, const double* D, int incremental, double* F);
 
+/* hfg_fock on device memory: H, D and F are device pointers (n x n on the context's device,
+ * for a batch the concatenated matrices), so a device-resident SCF step (DIIS, diagonalisation and
+ * density on the GPU) hands its density to the build and takes the Fock matrix back without
+ * host copies. Synchronous on return like every other call; incremental builds work as in
+ * hfg_fock, and the two calls may be mixed. Replaces the same digestion as hfg_fock. */
+int hfg_fock_device(hfg_ctx* ctx, const double* H, const double* D, int incremental, double* F);
+
 /* Forget the previous density (next hfg_fock is a full build). */
 int hfg_reset_incremental(hfg_ctx* ctx);
 
@@ -101,12 +108,27 @@ int hfg_energy(hfg_ctx* ctx, const double* D, double* E2);
  * calc_scf_gradient_rhf (hfscf_grad_ints.cpp:110-151, hfscf_grad_rhf.cpp:121-141). */
 int hfg_energy_backward(hfg_ctx* ctx, const double* D, double* grad);
 
-/* Test hook: output-basis block (s1 s2|s3 s4), laid out [a][b][c][d], no screening.
+/* Test hook: output-basis block (s1 s2|s3 s4), laid out [a][b][c][d], read out of the production
+ * J/K kernels with unit densities, so it follows the context's screening (create the context with
+ * schwarz = 0 for the unscreened block) and costs n3 * n4 full J/K builds.
  * Replaces: Erios::compute_contracted_shell_quartet for one quartet (hfscf_oseri.cpp:22-246). */
 int hfg_eri_shell_quartet(hfg_ctx* ctx, int s1, int s2, int s3, int s4, double* block);
 
 /* Cartesian Schwarz diagonal (pq|pq), ncart x ncart (hfscf's Erios::screen). */
 int hfg_schwarz(hfg_ctx* ctx, double* screen);
+
+/* Test hook: F_0..F_m(T) (n x (m+1), row-major) for n values T >= 0, evaluated on the device by
+ * the functions the class kernels use: the tabulated Taylor step with down/upward recursion
+ * (closed_form = 0, m <= 17) or the closed forms of F_0 and F_1 (closed_form = 1, m <= 1).
+ * Replaces: extramath::F_nu and the downward recursion at its call site (hfscf: src/math/gamma.cpp:18-24,
+ * src/integrals/hfscf_oseri.cpp:103-118), without hfscf's T >= 1e-8 floor, which the kernels apply
+ * where hfscf does (L = 0 quartets). */
+int hfg_boys_probe(hfg_ctx* ctx, int m, int n, const double* T, double* F, int closed_form);
+
+/* Measurement hook, no hfscf counterpart: `reps` launches of a plain copy kernel of `bytes` bytes
+ * (reads and writes `bytes` each) on `device`, average milliseconds per copy in *ms_per_copy.
+ * A kernel of known traffic against which a profiler's HBM byte counters are scaled. */
+int hfg_copy_calibration(int device, long long bytes, int reps, double* ms_per_copy);
 
 /* ---- Batching mode: many independent molecules in one context ----------------------------
  * Molecules are added one by one (basis as for hfg_set_basis, offsets within the molecule, and

@@ -47,7 +47,15 @@ typedef struct hfscf_scf_options {
     double density_tol;   /* density-weighted quartet screening of incremental builds (hfg_options) */
     int graph;            /* replay each J/K build configuration from a captured HIP graph */
     int autotune;         /* batched runs: autotune the class launch shapes after the first build */
-    int reserved[3];
+    int rebuild_every;    /* incremental builds: every rebuild_every-th build is a full one, which clears
+                             what density screening accumulates along the chain (0 = never; default 8) */
+    int device_step;      /* GPU backend, solo runs: 1 = the whole SCF step on the GPU (DIIS, F' = X^T F X,
+                             diagonalisation, density and energy in rocBLAS / rocSOLVER, the Fock build on
+                             device memory); 0 = the host loop (default 1) */
+    double polish_rms;    /* mixed precision (solo runs): once the DIIS rms is below this, continue from a
+                             full FP64 build with a fresh DIIS subspace and converge in FP64, so D, F,
+                             the energy and the gradient are those of the FP64 SCF (0 = mixed to the
+                             end; default 1e-6) */
 } hfscf_scf_options;
 
 typedef struct hfscf_scf_result {

@@ -55,6 +55,8 @@ def lib():
         L.orc_grad2e.argtypes = [C.c_void_p, _dp, C.c_int, _dp]
         L.orc_boys.argtypes = [C.c_int, C.c_double]
         L.orc_boys.restype = C.c_double
+        L.orc_set_threads.argtypes = [C.c_int]
+        L.orc_set_threads.restype = C.c_int
         _lib = L
     return _lib
 

@@ -828,4 +828,11 @@ void orc_grad2e(void* h, const double* D, int natom, double* grad) {
 
 double orc_boys(int m, double T) { return boys_ref(m, T); }
 
+// OpenMP threads of the oracle's parallel loops (timed CPU baseline); returns the previous count
+int orc_set_threads(int n) {
+    const int prev = omp_get_max_threads();
+    if (n > 0) omp_set_num_threads(n);
+    return prev;
+}
+
 }  // extern "C"

@@ -35,21 +35,30 @@ def seeded_density(n, seed):
 
 
 def cfg1():
-    mol = H.Molecule.from_xyz(os.path.join(HERE, "h2o.xyz"), "sto-3g")
-    orc = oracle.Oracle(mol.shells(), mol.geometry()[1], mol.pure, 1e-14, screen=False)
-    S, T, V = mol.oneint()
+    """Water / STO-3G, conventional SCF with stored ERIs (no screening), DIIS 6, 1e-10: from the
+    oracle alone. energies are electronic energies per iteration (hfscf_hf_solver.cpp:244-252);
+    F_first / F_last are the Fock matrices of the first and last builds and D_first / D_last
+    the densities they were built from."""
+    from oracle import basis_ref, oneint_ref, scf_ref
+
+    Z, X = basis_ref.read_xyz(os.path.join(HERE, "h2o.xyz"))
+    sh, pure, n = basis_ref.shells_for(Z, "sto-3g")
+    S, T, V = oneint_ref.oneint(sh, pure, Z, X)
     Hc = T + V
-    focks = []
+    orc = oracle.Oracle(sh, X, pure, 1e-14, screen=False)
+    dens, focks = [], []
 
     def jk(D):
         J, K = orc.jk(D)
+        dens.append(D.copy())
         focks.append(Hc + 2 * J - K)
         return J, K
 
-    r = H.rhf(mol, jk=jk, diis_range=6, energy_tol=1e-10, rms_tol=1e-10)
-    np.savez_compressed(os.path.join(HERE, "cfg1_h2o_sto3g.npz"), energy=r["energy"], energies=r["energies"],
-                        F_first=focks[0], F_last=focks[-1], D=r["D"], H=Hc, S=S)
-    print("cfg1", r["energy"], r["iterations"])
+    r = scf_ref.rhf(S, Hc, int(sum(Z)) // 2, jk, diis_range=6, energy_tol=1e-10, rms_tol=1e-10)
+    energy = r["energy_elec"] + oneint_ref.enuc(Z, X)
+    np.savez_compressed(os.path.join(HERE, "cfg1_h2o_sto3g.npz"), energy=energy, energies=np.array(r["energies"]),
+                        F_first=focks[0], F_last=focks[-1], D_first=dens[0], D_last=dens[-1], D=r["D"], H=Hc, S=S)
+    print("cfg1", energy, r["iterations"])
 
 
 def big(name, xyz, basis, rows_seed=21, diis=8):

@@ -91,7 +91,9 @@ def test_autotune_keeps_results():
 @pytest.mark.parametrize("precision", [H.FP64, H.MIXED])
 def test_batch_rhf_matches_solo(precision):
     mols = _clusters(12)
-    opts = dict(diis_range=6, energy_tol=1e-10, rms_tol=1e-9, incremental=1, precision=precision)
+    # no FP64 finish in the mixed solo runs: batches stay in one precision, and each molecule
+    # follows the solo trajectory of the same options
+    opts = dict(diis_range=6, energy_tol=1e-10, rms_tol=1e-9, incremental=1, precision=precision, polish_rms=0.0)
     res, st = H.batch_rhf(mols, gradient=True, graph=1, autotune=1, **opts)
     print(f"batch: {st}", flush=True)
     for k, m in enumerate(mols):
@@ -106,3 +108,25 @@ def test_batch_rhf_matches_solo(precision):
         # FP64: the same numbers. Mixed: the FP32 gradient kernels evaluated at two densities that
         # agree to the SCF tolerance differ by their FP32 rounding, about 1e-7 Eh/bohr
         assert dg <= (1e-8 if precision == H.FP64 else 2e-6)
+
+
+def test_batch_seeded_subset_all_converge():
+    # 64 clusters drawn with a fixed seed from the 512 of the batching workload, spanning
+    # n = 1..10 waters; FP64 density-difference builds with density screening and a full build
+    # every 8 iterations: every molecule converges, and a sample matches its solo run to 1e-9 Eh
+    frames = H.read_xyz_frames(os.path.join(GOLD, "batch_h2o.xyz"))
+    rng = np.random.default_rng(64)
+    idx = sorted(rng.choice(len(frames), size=64, replace=False).tolist())
+    mols = [H.Molecule.from_arrays(*frames[k], "6-31g*") for k in idx]
+    sizes = sorted({m.natom // 3 for m in mols})
+    assert sizes == list(range(1, 11)), sizes
+    opts = dict(diis_range=6, energy_tol=1e-10, rms_tol=1e-9, incremental=1)
+    res, st = H.batch_rhf(mols, gradient=True, graph=1, autotune=1, **opts)
+    conv = [r["converged"] for r in res]
+    print(f"batch of 64: {sum(conv)} converged, {st['iterations']} iterations, "
+          f"max molecule iterations {max(r['iterations'] for r in res)}", flush=True)
+    assert all(conv), [idx[k] for k, c in enumerate(conv) if not c]
+    for k in range(0, 64, 11):
+        solo = H.rhf(mols[k], **opts)
+        assert solo["converged"]
+        assert abs(res[k]["energy_final"] - solo["energy_final"]) <= 1e-9

@@ -125,8 +125,15 @@ def test_cfg1_fock_matrices_and_energy():
     mol = _mol("h2o.xyz", "sto-3g")
     gpu = H.rhf(mol, diis_range=6, energy_tol=1e-10, rms_tol=1e-10, schwarz=0)
     ctx = H.GpuContext(mol, schwarz=False)
-    F_last = ctx.fock(f["H"], f["D"])
-    print(f"cfg1: dE {gpu['energy'] - float(f['energy']):.2e}", flush=True)
+    # the oracle's Fock matrices of the first and last SCF iterations, at the densities they were
+    # built from
+    dF1 = np.abs(ctx.fock(f["H"], f["D_first"]) - f["F_first"]).max()
+    dF2 = np.abs(ctx.fock(f["H"], f["D_last"]) - f["F_last"]).max()
+    S, T, V = mol.oneint()
+    dH = np.abs(T + V - f["H"]).max()
+    print(f"cfg1: dE {gpu['energy'] - float(f['energy']):.2e}, |dF| first {dF1:.2e} last {dF2:.2e}, "
+          f"|dH| {dH:.2e}", flush=True)
+    assert dF1 < 1e-11 and dF2 < 1e-11 and dH < 1e-12
     assert abs(gpu["energy"] - float(f["energy"])) < 1e-9
     assert np.abs(gpu["energies"] - f["energies"][: len(gpu["energies"])]).max() < 1e-9
 
