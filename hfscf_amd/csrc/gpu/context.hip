@@ -13,8 +13,11 @@
 
 #include "boys.hpp"
 #include "context.hpp"
+#include "coop.hpp"
 #include "flopmodel.hpp"
 #include "launch_bounds.hpp"
+#include "misc.hpp"
+#include "private_segments.hpp"
 
 using namespace hfg;
 
@@ -53,7 +56,7 @@ void assign_streams(hfg_ctx* c, bool grad, const std::vector<double>& w) {
     double load[kSideStreams] = {0};
     for (size_t k : ord) {
         const ClassWork& cw = c->classes[k];
-        if (class_spills(cw.la, cw.lb, cw.lc, cw.ld, grad)) {
+        if (class_spills(cw.la, cw.lb, cw.lc, cw.ld, grad) && (grad || !cw.coop)) {
             map[k] = -1;
             continue;
         }
@@ -553,6 +556,31 @@ int setup_geometry(hfg_ctx* c) {
         std::vector<ClassWork> sorted;
         for (size_t k : ord) sorted.push_back(c->classes[k]);
         c->classes.swap(sorted);
+        // programs of the classes on the cooperative J/K kernel (when their LDS fits a workgroup)
+        std::vector<int4> words;
+        std::vector<std::pair<size_t, CoopHostProgram>> progs;
+        for (size_t k = 0; k < c->classes.size(); ++k) {
+            ClassWork& cw = c->classes[k];
+            cw.coop = false;
+            if (!jk_uses_coop(cw.la, cw.lb, cw.lc, cw.ld)) continue;
+            CoopHostProgram hp = make_coop_program(cw.la, cw.lb, cw.lc, cw.ld, c->pure);
+            cw.coop_lds[0] = hp.lds_bytes(false);
+            cw.coop_lds[1] = hp.lds_bytes(true);
+            if (cw.coop_lds[0] + kCoopStaticLds > (size_t)c->max_lds) continue;
+            const size_t base = words.size();
+            words.insert(words.end(), hp.words.begin(), hp.words.end());
+            progs.emplace_back(base, std::move(hp));
+            cw.coop = true;
+        }
+        if (!words.empty()) {
+            HFG_CHECK(c, c->d_coop.upload(words, c->stream));
+            size_t i = 0;
+            for (size_t k = 0; k < c->classes.size(); ++k)
+                if (c->classes[k].coop) {
+                    c->classes[k].coop_args = coop_args_on_device(progs[i].second, c->d_coop.p + progs[i].first);
+                    ++i;
+                }
+        }
         std::vector<double> wj, wg;
         for (const auto& cw : c->classes) {
             wj.push_back(cw.flops);
@@ -729,7 +757,6 @@ int enqueue_classes(hfg_ctx* c, hipStream_t main, bool dscreen, bool timed, Clas
     a.slab = buf.slab;
     a.dmax = c->d_dmax.p;
     a.dtol = dscreen ? c->opt.density_tol : 0.0;
-    const int mixed = c->opt.precision == HFG_MIXED ? 1 : 0;
     const int ncls = (int)c->classes.size(), nshared = shared_class_count(c, false);
     if (set != ClassSet::Shared) HFG_CHECK(c, launch_zero_ints(buf.queue + nshared, ncls - nshared, main));
     bool shared_zeroed = false;
@@ -747,7 +774,7 @@ int enqueue_classes(hfg_ctx* c, hipStream_t main, bool dscreen, bool timed, Clas
             if (!cw.ev_stop && (e = hipEventCreate(&cw.ev_stop)) != hipSuccess) return e;
             if ((e = hipEventRecord(cw.ev_start, st)) != hipSuccess) return e;
         }
-        e = launch_jk(cw.la, cw.lb, cw.lc, cw.ld, c->pure, mixed, b, (int)cw.items.size(), cw.jk_shape, st);
+        e = launch_jk_class(c, cw, b, cw.jk_shape, st);
         if (e == hipSuccess && timed) e = hipEventRecord(cw.ev_stop, st);
         return e;
     }, false, set, join_shared));
@@ -781,6 +808,7 @@ int enqueue_gather(hfg_ctx* c, hipStream_t main, const JkBuffers& buf) {
 // records work of that stream (other contexts may enqueue there concurrently) and its kernels
 // keep running on the one queue that holds their scratch.
 int run_jk(hfg_ctx* c, bool dscreen = false) {
+    if (int r = check_private_segments(c, false)) return r;
     const JkBuffers buf = own_buffers(c);
     if (!c->opt.graph) {
         HFG_CHECK(c, hipEventRecord(c->ev0, c->stream));
@@ -922,6 +950,29 @@ void layout_molecules(hfg_ctx* c) {
 
 }  // namespace
 
+hipError_t hfg::launch_jk_class(hfg_ctx* c, const ClassWork& cw, const JkArgs& a, LaunchShape sh, hipStream_t st) {
+    const int mixed = c->opt.precision == HFG_MIXED ? 1 : 0;
+    if (cw.coop) return launch_jk_coop(a, cw.coop_args, cw.coop_lds[mixed], mixed, (int)cw.items.size(), sh, st);
+    return launch_jk(cw.la, cw.lb, cw.lc, cw.ld, c->pure, mixed, a, (int)cw.items.size(), sh, st);
+}
+
+int hfg::check_private_segments(hfg_ctx* c, bool grad) {
+    const bool mixed = c->opt.precision == HFG_MIXED;
+    for (const ClassWork& cw : c->classes) {
+        if (!grad && cw.coop) continue;  // the cooperative kernel holds its tables in LDS
+        const int b = private_segment_bytes(grad, mixed, cw.la, cw.lb, cw.lc, cw.ld, c->pure);
+        if (b < 0 || b > kMaxPrivateBytes) {
+            char msg[200];
+            std::snprintf(msg, sizeof msg,
+                          "%s kernel of class (%d%d|%d%d) needs %d bytes of scratch per lane, above the %d this "
+                          "library dispatches",
+                          grad ? "gradient" : "J/K", cw.la, cw.lb, cw.lc, cw.ld, b, kMaxPrivateBytes);
+            return fail(c, HFG_ERR_UNSUPPORTED, b < 0 ? std::string("no private-segment entry for a class") : msg);
+        }
+    }
+    return HFG_OK;
+}
+
 extern "C" {
 
 void hfg_default_options(hfg_options* opt) {
@@ -957,6 +1008,8 @@ int hfg_create(hfg_ctx** out, int device, const hfg_options* opt) {
         hfg_destroy(c);
         return HFG_ERR_HIP;
     }
+    if (hipDeviceGetAttribute(&c->max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess)
+        c->max_lds = 65536;
     for (int k = 0; k < kSideStreams; ++k)
         if (hipStreamCreate(&c->side[k]) != hipSuccess ||
             hipEventCreateWithFlags(&c->ev_join[k], hipEventDisableTiming) != hipSuccess) {
@@ -1372,6 +1425,7 @@ void hfg_destroy(hfg_ctx* c) {
     c->d_gJ_off.release();
     c->d_gK_off.release();
     c->d_shell_atom.release();
+    c->d_coop.release();
     c->d_shell_ao.release();
     c->d_shell_nf.release();
     c->d_shell_mol.release();

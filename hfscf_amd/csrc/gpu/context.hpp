@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "../../../include/hfgpu.h"
+#include "coop_args.hpp"
 #include "kernels.hpp"
 
 namespace hfg {
@@ -52,6 +53,11 @@ struct ClassWork {
     LaunchShape jk_shape{0, 0};    // persistent workgroups / block of the J/K launch (0 = automatic)
     LaunchShape grad_shape{0, 0};  // the same for the gradient launch
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;  // brackets of the last direct launch
+    // J/K on the wavefront-cooperative kernel: its program (device pointers into the context's
+    // d_coop buffer) and the LDS bytes of its FP64 and mixed launches
+    bool coop = false;
+    CoopArgs coop_args{};
+    size_t coop_lds[2] = {0, 0};
 };
 
 // One molecule of a context (a context holds one, or many in batching mode).
@@ -84,6 +90,7 @@ struct hfg_ctx {
     using WorkItem = hfg::WorkItem;
     using GatherEntry = hfg::GatherEntry;
     int device = 0;
+    int max_lds = 65536;  // LDS bytes one workgroup may use on the device
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     hipStream_t side[hfg::kSideStreams] = {};
@@ -134,6 +141,7 @@ struct hfg_ctx {
     // workspace of the gradient classes whose recursion tables live in global memory (their
     // launches share it and run in order on the shared stream); allocated at the first gradient
     DevBuf<double> d_gws;
+    DevBuf<int4> d_coop;  // programs of the classes on the cooperative J/K kernel
     DevBuf<float> d_dmax;  // largest |density| per shell block (density-weighted screening)
     std::vector<float> dmax;
     DevBuf<double4> d_pp0, d_pp1, d_pp2;
@@ -230,6 +238,13 @@ inline hipStream_t class_launch_stream(hfg_ctx* c, size_t k, bool grad) {
     const int s = (grad ? c->grad_stream : c->class_stream)[k];
     return s < 0 ? c->heavy : c->side[s];
 }
+
+// The J/K launch of one class (the cooperative kernel or the class kernel) on stream st.
+hipError_t launch_jk_class(hfg_ctx* c, const ClassWork& cw, const JkArgs& a, LaunchShape sh, hipStream_t st);
+
+// HFG_OK when every class kernel that a J/K build (or a gradient when grad) dispatches has a
+// private segment within kMaxPrivateBytes; HFG_ERR_UNSUPPORTED with the class named otherwise.
+int check_private_segments(hfg_ctx* c, bool grad);
 
 // Run class k alone (J/K, or the gradient when grad) with launch shape sh on the main stream,
 // bracketed by ev0 / ev1, and wait for it (the autotuner's measurement).

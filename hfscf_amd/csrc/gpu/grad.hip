@@ -61,6 +61,7 @@ hipError_t ensure_grad_ws(hfg_ctx* c, int mixed) {
 }  // namespace
 
 int run_gradient(hfg_ctx* c, const double* D, double* grad) {
+    if (int r = check_private_segments(c, true)) return r;
     for (const auto& M : c->mols)
         if (M.natom > kMaxAtoms) {
             c->err = "hfg_energy_backward: more atoms in a molecule than the gradient kernel's LDS table holds";
@@ -105,6 +106,7 @@ int run_gradient(hfg_ctx* c, const double* D, double* grad) {
 }
 
 int run_one_class(hfg_ctx* c, size_t k, bool grad, LaunchShape sh) {
+    if (int r = check_private_segments(c, grad)) return r;
     const ClassWork& cw = c->classes[k];
     const int mixed = c->opt.precision == HFG_MIXED ? 1 : 0;
     int* q = c->d_queue.p + (grad ? c->classes.size() : 0) + k;
@@ -124,7 +126,7 @@ int run_one_class(hfg_ctx* c, size_t k, bool grad, LaunchShape sh) {
         JkArgs b = jk_args(c);
         b.items = c->d_items.p + cw.dev_off;
         b.queue = q;
-        HFG_CHECK(c, launch_jk(cw.la, cw.lb, cw.lc, cw.ld, c->pure, mixed, b, (int)cw.items.size(), sh, st));
+        HFG_CHECK(c, launch_jk_class(c, cw, b, sh, st));
     }
     HFG_CHECK(c, hipEventRecord(c->ev1, st));
     HFG_CHECK(c, hipStreamSynchronize(st));

@@ -74,12 +74,6 @@ hipError_t launch_screen(int la, int lb, const ScreenArgs& a, hipStream_t s);
 hipError_t launch_gather(const GatherArgs& a, int ntiles, hipStream_t s);
 // Sets n ints to zero (a kernel, so it may sit in a captured graph).
 hipError_t launch_zero_ints(int* p, int n, hipStream_t s);
-hipError_t launch_copy(const double2* src, double2* dst, size_t n, hipStream_t s);
-hipError_t launch_axpby(double a, const double* x, double b, const double* y, double* z, size_t n, hipStream_t s);
-hipError_t launch_block_absmax(const double* D, const MolDev* mols, const int* shell_ao, const int* shell_nf,
-                               const int* shell_mol, int nshell, float* dmax, hipStream_t s);
-hipError_t launch_boys_probe(int m, const double* T, const double* tab, double* out, int n, int closed_form,
-                             hipStream_t s);
 hipError_t launch_fock(const double* base, const double* J, const double* K, double* F, double* Facc, int nn,
                        int accumulate, hipStream_t s);
 

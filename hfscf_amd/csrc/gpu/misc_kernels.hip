@@ -1,5 +1,6 @@
 // Schwarz diagonal, gather, counter reset and Fock kernels.
 #include "jk_impl.hpp"
+#include "misc.hpp"
 
 namespace hfg {
 

@@ -101,3 +101,38 @@ def test_batch_geometry_frames():
         assert np.abs(d - 0.9572).max() < 1e-9
     mol = H.Molecule.from_arrays(*frames[0], "6-31g*")
     assert mol.nbf == 19 * (len(frames[0][0]) // 3)
+
+
+def _private_segments():
+    import sys
+    sys.path.insert(0, os.path.join(ROOT, "hfscf_amd", "csrc"))
+    import private_segments
+    return private_segments
+
+
+def _limit():
+    text = open(os.path.join(ROOT, "hfscf_amd", "csrc", "gpu", "private_segments.hpp")).read()
+    return int(re.search(r"kMaxPrivateBytes\s*=\s*(\d+)", text).group(1))
+
+
+def test_private_segments_within_dispatch_limit():
+    # every class kernel in the built library needs at most the per-lane scratch that the library
+    # dispatches (the largest that has run), and none uses a dynamic stack
+    P = _private_segments()
+    ks = P.library_kernels(os.path.join(ROOT, "hfscf_amd", "lib", "libhfgpu.so"))
+    cls = [k for k in ks if any(t in k[0] for t in ("jk_kernel", "jk_coop_kernel", "grad_kernel", "grad_ws_kernel"))]
+    assert sum("jk_coop_kernel" in k[0] for k in cls) == 2  # FP64 and mixed
+    assert len(cls) >= 150, len(cls)
+    limit = _limit()
+    over = [(name, b) for name, b, dyn, _ in cls if b > limit or dyn]
+    assert not over, over
+    print(f"{len(cls)} class kernels, largest private segment {max(k[1] for k in cls)} bytes (limit {limit})")
+
+
+def test_private_segment_table_matches_library():
+    # the table the library checks before a dispatch lists the same per-class maxima
+    text = open(os.path.join(ROOT, "hfscf_amd", "_build", "private_segments.cpp")).read()
+    rows = re.findall(r"\{(\d), (\d), (\d), (\d), (\d), (\d), (\d), (\d+), (\d)\}", text)
+    assert len(rows) >= 150
+    assert max(int(r[7]) for r in rows) <= _limit()
+    assert all(r[8] == "0" for r in rows)
