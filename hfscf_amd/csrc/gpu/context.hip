@@ -14,6 +14,7 @@
 #include "boys.hpp"
 #include "context.hpp"
 #include "coop.hpp"
+#include "coop_grad.hpp"
 #include "flopmodel.hpp"
 #include "launch_bounds.hpp"
 #include "misc.hpp"
@@ -56,7 +57,7 @@ void assign_streams(hfg_ctx* c, bool grad, const std::vector<double>& w) {
     double load[kSideStreams] = {0};
     for (size_t k : ord) {
         const ClassWork& cw = c->classes[k];
-        if (class_spills(cw.la, cw.lb, cw.lc, cw.ld, grad) && (grad || !cw.coop)) {
+        if (class_spills(cw.la, cw.lb, cw.lc, cw.ld, grad) && !(grad ? cw.gcoop : cw.coop)) {
             map[k] = -1;
             continue;
         }
@@ -572,6 +573,36 @@ int setup_geometry(hfg_ctx* c) {
             progs.emplace_back(base, std::move(hp));
             cw.coop = true;
         }
+        // gradient programs of the classes on the cooperative gradient kernel
+        std::vector<int4> gwords;
+        std::vector<std::pair<size_t, CoopGradHostProgram>> gprogs;
+        for (size_t k = 0; k < c->classes.size(); ++k) {
+            ClassWork& cw = c->classes[k];
+            cw.gcoop = false;
+            if (!grad_uses_coop(cw.la, cw.lb, cw.lc, cw.ld)) continue;
+            const size_t limit = (size_t)c->max_lds - kCoopGradStaticLds;
+            CoopGradHostProgram hp;
+            try {
+                hp = make_coop_grad_program(cw.la, cw.lb, cw.lc, cw.ld, c->pure, limit);
+            } catch (const std::exception&) {
+                continue;  // the class has no gradient kernel on this device (hfg_energy_backward refuses it)
+            }
+            cw.gcoop_lds[0] = hp.lds_bytes(false);
+            cw.gcoop_lds[1] = hp.lds_bytes(true);
+            const size_t base = gwords.size();
+            gwords.insert(gwords.end(), hp.words.begin(), hp.words.end());
+            gprogs.emplace_back(base, std::move(hp));
+            cw.gcoop = true;
+        }
+        if (!gwords.empty()) {
+            HFG_CHECK(c, c->d_gcoop.upload(gwords, c->stream));
+            size_t i = 0;
+            for (size_t k = 0; k < c->classes.size(); ++k)
+                if (c->classes[k].gcoop) {
+                    c->classes[k].gcoop_args = coop_grad_args_on_device(gprogs[i].second, c->d_gcoop.p + gprogs[i].first);
+                    ++i;
+                }
+        }
         if (!words.empty()) {
             HFG_CHECK(c, c->d_coop.upload(words, c->stream));
             size_t i = 0;
@@ -959,7 +990,10 @@ hipError_t hfg::launch_jk_class(hfg_ctx* c, const ClassWork& cw, const JkArgs& a
 int hfg::check_private_segments(hfg_ctx* c, bool grad) {
     const bool mixed = c->opt.precision == HFG_MIXED;
     for (const ClassWork& cw : c->classes) {
-        if (!grad && cw.coop) continue;  // the cooperative kernel holds its tables in LDS
+        if (grad ? cw.gcoop : cw.coop) continue;  // the cooperative kernels hold their tables in LDS
+        if ((grad ? grad_uses_coop(cw.la, cw.lb, cw.lc, cw.ld) : jk_uses_coop(cw.la, cw.lb, cw.lc, cw.ld)))
+            return fail(c, HFG_ERR_UNSUPPORTED, "the class needs the cooperative kernel, whose LDS does not fit a "
+                                                "workgroup on this device");
         const int b = private_segment_bytes(grad, mixed, cw.la, cw.lb, cw.lc, cw.ld, c->pure);
         if (b < 0 || b > kMaxPrivateBytes) {
             char msg[200];
@@ -1411,7 +1445,7 @@ void hfg_destroy(hfg_ctx* c) {
     c->d_dmax.release();
     c->d_grad_slab.release();
     c->d_grad.release();
-    c->d_gws.release();
+    c->d_gcoop.release();
     c->d_pp0.release();
     c->d_pp1.release();
     c->d_pp2.release();

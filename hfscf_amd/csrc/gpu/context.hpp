@@ -58,6 +58,10 @@ struct ClassWork {
     bool coop = false;
     CoopArgs coop_args{};
     size_t coop_lds[2] = {0, 0};
+    // the gradient on the cooperative kernel (program in d_gcoop)
+    bool gcoop = false;
+    CoopGradArgs gcoop_args{};
+    size_t gcoop_lds[2] = {0, 0};
 };
 
 // One molecule of a context (a context holds one, or many in batching mode).
@@ -138,10 +142,8 @@ struct hfg_ctx {
     double total_flops = 0, total_grad_flops = 0;
     // device
     DevBuf<double> d_psq, d_boys, d_D, d_J, d_K, d_F, d_H, d_G, d_slab, d_psmean, d_grad_slab, d_grad;
-    // workspace of the gradient classes whose recursion tables live in global memory (their
-    // launches share it and run in order on the shared stream); allocated at the first gradient
-    DevBuf<double> d_gws;
-    DevBuf<int4> d_coop;  // programs of the classes on the cooperative J/K kernel
+    DevBuf<int4> d_coop;   // programs of the classes on the cooperative J/K kernel
+    DevBuf<int4> d_gcoop;  // programs of the classes on the cooperative gradient kernel
     DevBuf<float> d_dmax;  // largest |density| per shell block (density-weighted screening)
     std::vector<float> dmax;
     DevBuf<double4> d_pp0, d_pp1, d_pp2;

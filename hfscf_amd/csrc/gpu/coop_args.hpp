@@ -47,4 +47,42 @@ struct CoopArgs {
     int off_bra, off_I, off_D, lds_r;
 };
 
+constexpr int kCoopGradBlock = 256;  // four wavefronts share a quartet of the gradient kernel
+constexpr int kCoopGradStaticLds = 6144;  // static LDS of the gradient kernel (atom sums, rows), bounded
+
+// One shifted class of a gradient program.
+struct CoopShift {
+    int pos, shift;              // shifted index (0 a, 1 b, 2 c) and +1 / -1
+    int na, nb, nc, nd;          // Cartesian functions of the shifted class
+    int NE, NF;                  // bra range (functions e with sa <= |e| <= sa + sb), ket range
+    const int4* hbops;           // bra HRR program (the form of CoopArgs)
+    int hblev[kCoopMaxLevels + 1];
+    int nhblev, hbsz, hb_chunk;  // hb_chunk ket functions at a time
+    const int* hb_init;          // entry of (e, 0) per bra function e
+    const int* hb_src;           // VRR entry of [e0|f0]^(0), index e * NF + f
+    const int2* hb_out;
+    int nhb_out;
+    const int4* hkops;
+    int hklev[kCoopMaxLevels + 1];
+    int nhklev, hksz, hk_chunk;
+    const int* hk_init;
+    const int2* hk_out;          // (c * nd + d, entry)
+    int nhk_out;
+    const int* cg;               // per shifted integral and direction: G entry | factor << 20, or -1
+};
+
+struct CoopGradArgs {
+    int la, lb, lc, ld, LT;      // unshifted class; the VRR runs at LT + 1
+    int NA, NB, NC, ND, na, nb, nc, nd;
+    int pure;
+    const int4* vops;
+    int vlev[kCoopMaxLevels + 1];
+    int nvlev;
+    int nshift;
+    CoopShift sh[6];
+    // LDS regions in R units: G (Cartesian density), V (VRR table; first the back-transform
+    // temporaries, 2 x the Cartesian block), H (bra HRR tables, then ket HRR tables), bra output
+    int off_V, off_Hb, off_bra, off_Hk, lds_r;
+};
+
 }  // namespace hfg

@@ -3,7 +3,7 @@
 #include <vector>
 
 #include "context.hpp"
-#include "grad_ws.hpp"
+#include "coop_grad.hpp"
 
 namespace hfg {
 
@@ -51,11 +51,11 @@ JkArgs jk_args(hfg_ctx* c) {
     return a;
 }
 
-// Workspace for the classes of this context that take the workspace kernel (none: no buffer).
-hipError_t ensure_grad_ws(hfg_ctx* c, int mixed) {
-    size_t need = 0;
-    for (const auto& cw : c->classes) need = std::max(need, grad_ws_doubles(cw.la, cw.lb, cw.lc, cw.ld, c->pure, mixed));
-    return need ? c->d_gws.ensure(need) : hipSuccess;
+// The gradient launch of one class: the cooperative kernel (L >= 7) or the class kernel.
+hipError_t launch_grad_any(hfg_ctx* c, const ClassWork& cw, const GradArgs& b, LaunchShape sh, int mixed,
+                           hipStream_t st) {
+    if (cw.gcoop) return launch_grad_coop(b, cw.gcoop_args, cw.gcoop_lds[mixed], mixed, (int)cw.items.size(), sh, st);
+    return launch_grad(cw.la, cw.lb, cw.lc, cw.ld, c->pure, mixed, b, (int)cw.items.size(), sh, st);
 }
 
 }  // namespace
@@ -75,15 +75,13 @@ int run_gradient(hfg_ctx* c, const double* D, double* grad) {
     HFG_CHECK(c, c->d_grad.ensure((size_t)std::max(nout, 1)));
     GradArgs a = grad_args(c);
     const int mixed = c->opt.precision == HFG_MIXED ? 1 : 0;
-    HFG_CHECK(c, ensure_grad_ws(c, mixed));
     HFG_CHECK(c, hipEventRecord(c->ev0, c->stream));
     HFG_CHECK(c, launch_zero_ints(c->d_queue.p + ncls, ncls, c->stream));
     HFG_CHECK(c, for_each_class_concurrent(c, c->stream, [&](const ClassWork& cw, hipStream_t st, int qi) {
         GradArgs b = a;
         b.items = c->d_items.p + cw.dev_off;
         b.queue = c->d_queue.p + ncls + qi;
-        return launch_grad(cw.la, cw.lb, cw.lc, cw.ld, c->pure, mixed, b, c->d_gws.p, (int)cw.items.size(),
-                           cw.grad_shape, st);
+        return launch_grad_any(c, cw, b, cw.grad_shape, mixed, st);
     }, true));
     HFG_CHECK(c, hipEventRecord(c->ev1, c->stream));
     GradReduceArgs r{};
@@ -116,12 +114,11 @@ int run_one_class(hfg_ctx* c, size_t k, bool grad, LaunchShape sh) {
     HFG_CHECK(c, hipEventRecord(c->ev0, st));
     if (grad) {
         HFG_CHECK(c, c->d_grad_slab.ensure((size_t)std::max<long long>(c->grad_slab_doubles, 1)));
-        HFG_CHECK(c, ensure_grad_ws(c, mixed));
         GradArgs b = grad_args(c);
         b.slab = c->d_grad_slab.p;
         b.items = c->d_items.p + cw.dev_off;
         b.queue = q;
-        HFG_CHECK(c, launch_grad(cw.la, cw.lb, cw.lc, cw.ld, c->pure, mixed, b, c->d_gws.p, (int)cw.items.size(), sh, st));
+        HFG_CHECK(c, launch_grad_any(c, cw, b, sh, mixed, st));
     } else {
         JkArgs b = jk_args(c);
         b.items = c->d_items.p + cw.dev_off;

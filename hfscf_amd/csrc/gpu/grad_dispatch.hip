@@ -1,5 +1,5 @@
 // Run-time dispatch of the adjoint gradient kernels and the fixed-order slab reduction.
-#include "grad_ws.hpp"
+#include "coop_grad.hpp"
 #include "launch_bounds.hpp"
 
 namespace hfg {
@@ -8,9 +8,7 @@ namespace hfg {
     template <>                                                                                     \
     hipError_t launch_grad_class64<A, B, C, D, P>(const GradArgs&, int, LaunchShape, hipStream_t);  \
     template <>                                                                                     \
-    hipError_t launch_grad_class32<A, B, C, D, P>(const GradArgs&, int, LaunchShape, hipStream_t);  \
-    template <>                                                                                     \
-    hipError_t launch_grad_ws<A, B, C, D, P>(const GradArgs&, double*, int, hipStream_t);
+    hipError_t launch_grad_class32<A, B, C, D, P>(const GradArgs&, int, LaunchShape, hipStream_t);
 #define HFG_DECL(A, B, C, D) HFG_DECL1(A, B, C, D, true) HFG_DECL1(A, B, C, D, false)
 HFG_FOR_CLASSES(HFG_DECL)
 #undef HFG_DECL
@@ -18,18 +16,12 @@ HFG_FOR_CLASSES(HFG_DECL)
 
 namespace {
 
-// The workspace kernel for the FP64 classes that need it and for the mixed (dd|dd) class; the
-// register kernels (FP64 or FP32) for the others.
+// The register kernels (FP64 or FP32) of the classes below L = 7; the classes from L = 7 up run
+// the cooperative kernel (coop_grad.hpp), launched by the context, and have no register kernel.
 template <int A, int B, int C, int D, bool P>
-hipError_t grad_class(const GradArgs& a, double* ws, int nitems, LaunchShape sh, int mixed, hipStream_t s) {
-    if constexpr (grad_uses_ws(A, B, C, D)) {
-        if (ws == nullptr) return hipErrorInvalidValue;
-        if constexpr (A + B + C + D == 8) {
-            return launch_grad_ws<A, B, C, D, P>(a, ws, nitems, s);
-        } else {
-            return mixed ? launch_grad_class32<A, B, C, D, P>(a, nitems, scratch_safe_shape(A, B, C, D, true, sh, nitems), s)
-                         : launch_grad_ws<A, B, C, D, P>(a, ws, nitems, s);
-        }
+hipError_t grad_class(const GradArgs& a, int nitems, LaunchShape sh, int mixed, hipStream_t s) {
+    if constexpr (grad_uses_coop(A, B, C, D)) {
+        return hipErrorInvalidValue;
     } else {
         const LaunchShape bs = scratch_safe_shape(A, B, C, D, true, sh, nitems);
         return mixed ? launch_grad_class32<A, B, C, D, P>(a, nitems, bs, s)
@@ -39,28 +31,16 @@ hipError_t grad_class(const GradArgs& a, double* ws, int nitems, LaunchShape sh,
 
 }  // namespace
 
-size_t grad_ws_doubles(int la, int lb, int lc, int ld, bool pure, int mixed) {
-    if (!grad_uses_ws(la, lb, lc, ld)) return 0;
-    if (mixed && la + lb + lc + ld < 8) return 0;
-    return (size_t)ws_layout(la, lb, lc, ld, pure).total * kWsLanes;
-}
-
-hipError_t launch_grad(int la, int lb, int lc, int ld, bool pure, int mixed, const GradArgs& a, double* ws,
-                       int nitems, LaunchShape want, hipStream_t s) {
+hipError_t launch_grad(int la, int lb, int lc, int ld, bool pure, int mixed, const GradArgs& a, int nitems,
+                       LaunchShape want, hipStream_t s) {
     if (nitems == 0) return hipSuccess;
 #define HFG_CASE(A, B, C, D)                                                                        \
     if (la == A && lb == B && lc == C && ld == D)                                                   \
-        return pure ? grad_class<A, B, C, D, true>(a, ws, nitems, want, mixed, s)                   \
-                    : grad_class<A, B, C, D, false>(a, ws, nitems, want, mixed, s);
+        return pure ? grad_class<A, B, C, D, true>(a, nitems, want, mixed, s)                       \
+                    : grad_class<A, B, C, D, false>(a, nitems, want, mixed, s);
     HFG_FOR_CLASSES(HFG_CASE)
 #undef HFG_CASE
     return hipErrorInvalidValue;
-}
-
-// Without a workspace: the classes that need one report hipErrorInvalidValue.
-hipError_t launch_grad(int la, int lb, int lc, int ld, bool pure, int mixed, const GradArgs& a, int nitems,
-                       LaunchShape want, hipStream_t s) {
-    return launch_grad(la, lb, lc, ld, pure, mixed, a, nullptr, nitems, want, s);
 }
 
 // out[k] = sum over the work items of k's molecule of their partial k, summed in a fixed order

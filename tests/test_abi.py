@@ -120,8 +120,9 @@ def test_private_segments_within_dispatch_limit():
     # dispatches (the largest that has run), and none uses a dynamic stack
     P = _private_segments()
     ks = P.library_kernels(os.path.join(ROOT, "hfscf_amd", "lib", "libhfgpu.so"))
-    cls = [k for k in ks if any(t in k[0] for t in ("jk_kernel", "jk_coop_kernel", "grad_kernel", "grad_ws_kernel"))]
+    cls = [k for k in ks if any(t in k[0] for t in ("jk_kernel", "jk_coop_kernel", "grad_kernel", "grad_coop_kernel"))]
     assert sum("jk_coop_kernel" in k[0] for k in cls) == 2  # FP64 and mixed
+    assert sum("grad_coop_kernel" in k[0] for k in cls) == 2
     assert len(cls) >= 150, len(cls)
     limit = _limit()
     over = [(name, b) for name, b, dyn, _ in cls if b > limit or dyn]

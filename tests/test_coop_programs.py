@@ -63,3 +63,36 @@ def test_coop_programs_against_oracle(evaluator, basis, pure):
         assert 0 < lds <= 64 * 1024 - 2048  # FP64 LDS of the launch fits a 64 KB workgroup
         worst = max(worst, float(np.abs(out.reshape(nf) - o.shell_quartet(*q)).max()))
     assert worst < 1e-14, worst
+
+
+def test_coop_gradient_programs_against_oracle(evaluator):
+    # the cooperative gradient kernel's programs (VRR at L+1, the six shifted classes' HRR and
+    # contraction tables, the 2 alpha weights) against the oracle's derivative integrals
+    # (pinned by finite differences in test_oracle.py) contracted with a random Cartesian density
+    Z, X = basis_ref.read_xyz(golden("h2o_8.xyz"))
+    Z, X = Z[:6], X[:6]
+    sh, pure, n = basis_ref.shells_for(Z, "cc-pvdz", False)
+    o = oracle.Oracle(sh, X, False, 1e-30, screen=False)
+    L = sh["L"]
+    po = np.concatenate([[0], np.cumsum(sh["nprim"])])
+    d = [i for i in range(len(L)) if L[i] == 2]
+    p = [i for i in range(len(L)) if L[i] == 1]
+    qs = [(d[0], d[1], d[1], d[0]), (d[0], d[1], d[0], p[0]), (d[1], p[1], d[0], d[1]), (d[0], d[0], d[1], p[2]),
+          (d[1], p[0], d[0], p[3]), (d[0], d[1], p[2], p[1]), (d[0], 0, d[1], d[1])]
+    dp, ip = C.POINTER(C.c_double), C.POINTER(C.c_int)
+    rng = np.random.default_rng(0)
+    for q in qs:
+        Ls = [int(L[s]) for s in q]
+        nc = [(l + 1) * (l + 2) // 2 for l in Ls]
+        cen = np.ascontiguousarray(np.array([X[sh["atom"][s]] for s in q]).reshape(-1))
+        np4 = np.array([sh["nprim"][s] for s in q], dtype=np.int32)
+        al = np.concatenate([sh["alpha"][po[s]:po[s + 1]] for s in q])
+        co = np.concatenate([sh["coef"][po[s]:po[s + 1]] for s in q])
+        G = np.ascontiguousarray(rng.standard_normal(nc))
+        out = np.zeros(9)
+        lds = evaluator.coop_grad_eval(*Ls, cen.ctypes.data_as(dp), np4.ctypes.data_as(ip), al.ctypes.data_as(dp),
+                                       co.ctypes.data_as(dp), G.ctypes.data_as(dp), out.ctypes.data_as(dp))
+        assert 0 < lds <= 160 * 1024 - 6144  # fits a gfx950 workgroup next to the static LDS
+        der = o.shell_quartet_deriv1(*q)
+        ref = np.array([np.sum(G * der[k]) for k in range(9)])
+        assert np.abs(out - ref).max() <= 1e-14 * max(1.0, np.abs(ref).max()), (q, out - ref)
