@@ -101,6 +101,7 @@ def hfgpu():
         lib.hfg_schwarz.argtypes = [C.c_void_p, _dp]
         lib.hfg_copy_calibration.argtypes = [C.c_int, C.c_longlong, C.c_int, _dp]
         lib.hfg_boys_probe.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp, C.c_int]
+        lib.hfg_build_jk_noalloc.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         lib.hfg_last_stats.argtypes = [C.c_void_p, C.POINTER(HfgStats)]
         lib.hfg_nbf.argtypes = [C.c_void_p]
         lib.hfg_last_error.argtypes = [C.c_void_p]

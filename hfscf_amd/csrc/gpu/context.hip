@@ -18,6 +18,7 @@
 #include "flopmodel.hpp"
 #include "launch_bounds.hpp"
 #include "misc.hpp"
+#include "noalloc.hpp"
 #include "private_segments.hpp"
 
 using namespace hfg;
@@ -603,6 +604,9 @@ int setup_geometry(hfg_ctx* c) {
                     ++i;
                 }
         }
+        // the largest dynamic LDS any context of the process may request (the attribute is per
+        // kernel, so every context sets the same value)
+        HFG_CHECK(c, coop_set_lds_limit((size_t)c->max_lds - kCoopStaticLds, (size_t)c->max_lds - kCoopGradStaticLds));
         if (!words.empty()) {
             HFG_CHECK(c, c->d_coop.upload(words, c->stream));
             size_t i = 0;
@@ -1044,6 +1048,10 @@ int hfg_create(hfg_ctx** out, int device, const hfg_options* opt) {
     }
     if (hipDeviceGetAttribute(&c->max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess)
         c->max_lds = 65536;
+    if (noalloc_scratch(device, &c->scratch) != hipSuccess) {  // hfg_build_jk_noalloc's fixed scratch
+        hfg_destroy(c);
+        return HFG_ERR_HIP;
+    }
     for (int k = 0; k < kSideStreams; ++k)
         if (hipStreamCreate(&c->side[k]) != hipSuccess ||
             hipEventCreateWithFlags(&c->ev_join[k], hipEventDisableTiming) != hipSuccess) {
@@ -1148,6 +1156,30 @@ int hfg_build_jk(hfg_ctx* c, const double* D, double* J, double* K, int flags) {
     if ((r = run_jk(c))) return r;
     if (J) HFG_CHECK(c, hipMemcpyAsync(J, c->d_J.p, nn * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     if (K) HFG_CHECK(c, hipMemcpyAsync(K, c->d_K.p, nn * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    return finish_stats(c, t0);
+}
+
+int hfg_build_jk_noalloc(hfg_ctx* c, const double* D, double* J, double* K) {
+    int r = check_ready(c);
+    if (r) return r;
+    if (!D || !J || !K) return fail(c, HFG_ERR_ARG, "hfg_build_jk_noalloc: NULL argument");
+    const size_t ncls = c->classes.size();
+    const size_t need = (size_t)c->slab_doubles;
+    if (ncls > 2 * kNoallocCounterDoubles || need > kNoallocScratchDoubles - kNoallocCounterDoubles) {
+        char msg[200];
+        std::snprintf(msg, sizeof msg,
+                      "hfg_build_jk_noalloc: the build needs %.1f MiB of scratch, the fixed maximum is %.1f MiB "
+                      "(use hfg_build_jk)",
+                      (need + kNoallocCounterDoubles) * 8.0 / 1048576.0, kNoallocScratchDoubles * 8.0 / 1048576.0);
+        return fail(c, HFG_ERR_ARG, msg);
+    }
+    if ((r = check_private_segments(c, false))) return r;
+    auto t0 = std::chrono::steady_clock::now();
+    const JkBuffers buf{D, J, K, c->scratch + kNoallocCounterDoubles, reinterpret_cast<int*>(c->scratch)};
+    HFG_CHECK(c, hipEventRecord(c->ev0, c->stream));
+    if ((r = enqueue_classes(c, c->stream, false, false, ClassSet::All, buf))) return r;
+    HFG_CHECK(c, hipEventRecord(c->ev1, c->stream));
+    if ((r = enqueue_gather(c, c->stream, buf))) return r;
     return finish_stats(c, t0);
 }
 

@@ -95,6 +95,7 @@ struct hfg_ctx {
     using GatherEntry = hfg::GatherEntry;
     int device = 0;
     int max_lds = 65536;  // LDS bytes one workgroup may use on the device
+    double* scratch = nullptr;  // the device's fixed scratch of hfg_build_jk_noalloc
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     hipStream_t side[hfg::kSideStreams] = {};

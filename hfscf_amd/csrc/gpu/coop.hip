@@ -213,18 +213,25 @@ hipError_t launch_jk_coop(const JkArgs& a, const CoopArgs& p, size_t lds_bytes, 
     const int grid = sh.grid > 0 ? std::min(sh.grid, nitems) : nitems;
     if (mixed) {
         auto k = jk_coop_kernel<float, double>;
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           (int)lds_bytes);
-        if (e != hipSuccess) return e;
         hipLaunchKernelGGL(k, dim3(grid), dim3(kCoopBlock), lds_bytes, s, b, p);
     } else {
         auto k = jk_coop_kernel<double, double>;
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           (int)lds_bytes);
-        if (e != hipSuccess) return e;
         hipLaunchKernelGGL(k, dim3(grid), dim3(kCoopBlock), lds_bytes, s, b, p);
     }
     return hipGetLastError();
+}
+
+hipError_t coop_set_lds_limit(size_t jk_bytes, size_t grad_bytes) {
+    // the dynamic LDS the launches may request, set once (not while a stream is being captured)
+    hipError_t e = hipSuccess;
+    auto set = [&](const void* k, size_t b) {
+        if (e == hipSuccess && b > 0) e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)b);
+    };
+    set(reinterpret_cast<const void*>(jk_coop_kernel<double, double>), jk_bytes);
+    set(reinterpret_cast<const void*>(jk_coop_kernel<float, double>), jk_bytes);
+    set(reinterpret_cast<const void*>(grad_coop_kernel<double>), grad_bytes);
+    set(reinterpret_cast<const void*>(grad_coop_kernel<float>), grad_bytes);
+    return e;
 }
 
 // ---------------------------------------------------------------------- gradient programs
@@ -381,15 +388,9 @@ hipError_t launch_grad_coop(const GradArgs& a, const CoopGradArgs& p, size_t lds
     const int grid = sh.grid > 0 ? std::min(sh.grid, nitems) : nitems;
     if (mixed) {
         auto k = grad_coop_kernel<float>;
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           (int)lds_bytes);
-        if (e != hipSuccess) return e;
         hipLaunchKernelGGL(k, dim3(grid), dim3(kCoopGradBlock), lds_bytes, s, b, p);
     } else {
         auto k = grad_coop_kernel<double>;
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           (int)lds_bytes);
-        if (e != hipSuccess) return e;
         hipLaunchKernelGGL(k, dim3(grid), dim3(kCoopGradBlock), lds_bytes, s, b, p);
     }
     return hipGetLastError();

@@ -380,5 +380,8 @@ CoopArgs coop_args_on_device(const CoopHostProgram& h, const int4* dev);
 
 hipError_t launch_jk_coop(const JkArgs& a, const CoopArgs& p, size_t lds_bytes, int mixed, int nitems,
                           LaunchShape sh, hipStream_t s);
+// The largest dynamic LDS the cooperative J/K and gradient launches request (set once per process
+// before any launch; the attribute is per kernel).
+hipError_t coop_set_lds_limit(size_t jk_bytes, size_t grad_bytes);
 
 }  // namespace hfg

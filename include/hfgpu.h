@@ -78,6 +78,16 @@ int hfg_set_geometry(hfg_ctx* ctx, int natom, const double* xyz_bohr, const int*
 /* J and K (nbf x nbf) for the half density D. Either output may be NULL. flags reserved (0). */
 int hfg_build_jk(hfg_ctx* ctx, const double* D, double* J, double* K, int flags);
 
+/* hfg_build_jk on device memory without any allocation. No counterpart in hfscf: for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). D, J and K are device pointers (nbf x nbf, for a batch the
+ * concatenated matrices) on the context's device; the kernels' queue counters and tile slab live
+ * in a fixed 32 MiB array per GPU, so a build whose tiles need more is refused (HFG_ERR_ARG,
+ * the message gives the size; larger molecules take hfg_build_jk). Every call
+ * on a GPU shares that array: calls that overlap on one GPU (threads, contexts) race. The caller
+ * makes D ready before the call; the call is synchronous on return. */
+int hfg_build_jk_noalloc(hfg_ctx* ctx, const double* D, double* J, double* K);
+
 /* Fock matrix F = H + 2J - K. When incremental != 0 and a previous build exists, J and K are
  * built from D - D_prev and added to the previous two-electron matrix (the FP32 error of the
  * mixed mode then shrinks with the density change). Replaces: hf_solver::update_fock_matrix's
