@@ -83,7 +83,7 @@ def cpu_model():
     return "unknown"
 
 
-def cpu_baseline(mol, rows=6, threads=None):
+def cpu_baseline(mol, rows=12, threads=None):
     """The oracle (CPU restatement of hfscf's stored-ERI path, OpenMP) on this host's cores: the ERI
     build of `rows` shell rows taken at the midpoints of equal strata of the shell index (rows
     hold unequal work, so the midpoint of each stratum, not its first row), scaled by the stride,
@@ -92,7 +92,9 @@ def cpu_baseline(mol, rows=6, threads=None):
     import oracle
 
     lib = oracle.lib()
-    nthreads = threads or os.cpu_count() or 1
+    # the host's share of cores for this GPU: OMP_NUM_THREADS where the environment sets it,
+    # else the visible cores up to 16
+    nthreads = threads or int(os.environ.get("OMP_NUM_THREADS") or 0) or min(os.cpu_count() or 1, 16)
     lib.orc_set_threads(nthreads)
     orc = oracle.Oracle(mol.shells(), mol.geometry()[1], mol.pure, 1e-14, screen=True)
     ns = mol.nshell

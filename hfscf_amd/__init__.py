@@ -88,6 +88,7 @@ def hfgpu():
         lib.hfg_fock.argtypes = [C.c_void_p, _dp, _dp, C.c_int, _dp]
         lib.hfg_reset_incremental.argtypes = [C.c_void_p]
         lib.hfg_set_precision.argtypes = [C.c_void_p, C.c_int]
+        lib.hfg_set_density_tol.argtypes = [C.c_void_p, C.c_double]
         lib.hfg_batch_add_molecule.argtypes = [C.c_void_p, C.c_int, _ip, _ip, _ip, _dp, _dp, C.c_int, _ip, _ip,
                                                C.c_int, _dp]
         lib.hfg_batch_commit.argtypes = [C.c_void_p]
@@ -452,6 +453,9 @@ class GpuContext:
 
     def set_precision(self, precision):
         self._check(hfgpu().hfg_set_precision(self._h, int(precision)))
+
+    def set_density_tol(self, density_tol):
+        self._check(hfgpu().hfg_set_density_tol(self._h, float(density_tol)))
 
     def energy(self, D):
         e = C.c_double()

@@ -853,9 +853,10 @@ int run_jk(hfg_ctx* c, bool dscreen = false) {
         return enqueue_gather(c, c->stream, buf);
     }
     const int mixed = c->opt.precision == HFG_MIXED ? 1 : 0;
+    const double dtol = dscreen ? c->opt.density_tol : 0.0;
     JkGraph* gr = nullptr;
     for (auto& g : c->graphs)
-        if (g.valid && g.mixed == mixed && g.dscreen == dscreen) gr = &g;
+        if (g.valid && g.mixed == mixed && g.dscreen == dscreen && g.dtol == dtol) gr = &g;
     if (!gr) {
         hipGraph_t graph = nullptr;
         HFG_CHECK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
@@ -872,6 +873,7 @@ int run_jk(hfg_ctx* c, bool dscreen = false) {
         ng.valid = true;
         ng.mixed = mixed;
         ng.dscreen = dscreen;
+        ng.dtol = dtol;
         c->graphs.push_back(ng);
         gr = &c->graphs.back();
     }
@@ -1253,6 +1255,13 @@ int hfg_set_precision(hfg_ctx* c, int precision) {
     if (!c) return HFG_ERR_ARG;
     if (precision != HFG_FP64 && precision != HFG_MIXED) return fail(c, HFG_ERR_ARG, "hfg_set_precision: unknown mode");
     c->opt.precision = precision;
+    return HFG_OK;
+}
+
+int hfg_set_density_tol(hfg_ctx* c, double density_tol) {
+    if (!c) return HFG_ERR_ARG;
+    if (!(density_tol >= 0.0)) return fail(c, HFG_ERR_ARG, "hfg_set_density_tol: the threshold must be >= 0");
+    c->opt.density_tol = density_tol;
     return HFG_OK;
 }
 

@@ -81,6 +81,7 @@ struct JkGraph {
     bool valid = false;
     int mixed = 0;
     bool dscreen = false;
+    double dtol = 0.0;  // the density threshold baked into the captured launch arguments
     hipGraphExec_t exec = nullptr;
 };
 

@@ -130,6 +130,10 @@ bool hf_solver::scf_run_device() {
     double* eslot = dv.alloc<double>(nn * nslot);
     double* fslot = dv.alloc<double>(nn * nslot);
     rocblas_int* info = dv.alloc<rocblas_int>(1);
+    // the extrapolation is one product over every slot with zero weights on the unused ones, so
+    // the slots start at zero: 0 x (whatever hipMalloc's memory held, NaN included) is not 0
+    hcheck(hipMemsetAsync(eslot, 0, nn * nslot * sizeof(double), dv.st), "hipMemsetAsync");
+    hcheck(hipMemsetAsync(fslot, 0, nn * nslot * sizeof(double), dv.st), "hipMemsetAsync");
     dv.upload(S, S_.data(), nn);
     dv.upload(H, H_.data(), nn);
 

@@ -58,10 +58,21 @@ hfg_options gpu_options(const hfscf_scf_options& o) {
     return go;
 }
 
+// Density threshold of one incremental build. The FP32 density-difference builds of a mixed run
+// keep the caller's density_tol (their rounding error is the larger one). An FP64 build must not
+// add more error than the energy tolerance allows, and the contributions of the quartets it
+// skips add up over the build: on a 384-function water cluster a threshold t left SCF energies
+// that wandered by a few hundred to a few thousand t from one iteration to the next. FP64
+// builds therefore use at most 1e-5 x energy_tol.
+double build_density_tol(const hfscf_scf_options& o, bool fp32_build) {
+    return fp32_build ? o.density_tol : std::min(o.density_tol, 1e-5 * o.energy_tol);
+}
+
 // GPU backend: ERIOS/hf_solver/scf_gradient two-electron work through the hfgpu C ABI.
 class GpuJK final : public JKBuilder {
    public:
-    GpuJK(const Molecule& mol, const hfscf_scf_options& o) : mol_(mol), precision_(o.precision), device_(o.device) {
+    GpuJK(const Molecule& mol, const hfscf_scf_options& o)
+        : mol_(mol), opt_(o), precision_(o.precision), device_(o.device) {
         const hfg_options go = gpu_options(o);
         int r = hfg_create(&ctx_, o.device, &go);
         if (r) throw std::runtime_error("hfg_create failed (" + std::to_string(r) + "): no usable GPU");
@@ -73,16 +84,12 @@ class GpuJK final : public JKBuilder {
     ~GpuJK() override { hfg_destroy(ctx_); }
     void build_jk(const double* D, double* J, double* K) override { check(hfg_build_jk(ctx_, D, J, K, 0)); }
     void fock(const double* H, const double* D, int incremental, double* F) override {
-        // mixed precision: FP32 only for builds on a density difference, whose FP32 error
-        // shrinks with the difference; full builds (the first, the final one) in FP64
-        const bool inc = incremental && have_prev_;
-        check(hfg_set_precision(ctx_, (precision_ == HFG_MIXED && inc && !final_ && !fp64_) ? HFG_MIXED : HFG_FP64));
+        prepare(incremental);
         check(hfg_fock(ctx_, H, D, incremental, F));
         have_prev_ = true;
     }
     bool fock_device(const double* H, const double* D, int incremental, double* F) override {
-        const bool inc = incremental && have_prev_;
-        check(hfg_set_precision(ctx_, (precision_ == HFG_MIXED && inc && !final_ && !fp64_) ? HFG_MIXED : HFG_FP64));
+        prepare(incremental);
         check(hfg_fock_device(ctx_, H, D, incremental, F));
         have_prev_ = true;
         return true;
@@ -105,7 +112,16 @@ class GpuJK final : public JKBuilder {
     void check(int r) {
         if (r) throw std::runtime_error(std::string("hfgpu: ") + hfg_last_error(ctx_));
     }
+    // mixed precision: FP32 only for builds on a density difference, whose FP32 error shrinks
+    // with the difference; full builds (the first, the final one) and the FP64 finish in FP64
+    void prepare(int incremental) {
+        const bool inc = incremental && have_prev_;
+        const bool fp32 = precision_ == HFG_MIXED && inc && !final_ && !fp64_;
+        check(hfg_set_precision(ctx_, fp32 ? HFG_MIXED : HFG_FP64));
+        check(hfg_set_density_tol(ctx_, build_density_tol(opt_, fp32)));
+    }
     const Molecule& mol_;
+    const hfscf_scf_options opt_;
     hfg_ctx* ctx_ = nullptr;
     int precision_;
     int device_;
@@ -216,7 +232,9 @@ int batch_rhf(int nmol, hfscf_mol* const* mols, const hfscf_scf_options& opt, hf
         const int inc = (opt.incremental && it > chain_start &&
                          (opt.rebuild_every <= 0 || (it - chain_start) % opt.rebuild_every != 0)) ? 1 : 0;
         if (!inc) chain_start = it;
-        check(hfg_set_precision(ctx, (opt.precision == HFG_MIXED && inc) ? HFG_MIXED : HFG_FP64));
+        const bool fp32 = opt.precision == HFG_MIXED && inc;
+        check(hfg_set_precision(ctx, fp32 ? HFG_MIXED : HFG_FP64));
+        check(hfg_set_density_tol(ctx, build_density_tol(opt, fp32)));
         const auto t0 = clk::now();
         check(hfg_fock(ctx, H.data(), D.data(), inc, F.data()));
         const double tf = secs(t0, clk::now());

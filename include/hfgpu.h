@@ -109,6 +109,11 @@ int hfg_reset_incremental(hfg_ctx* ctx);
  * on the converged density; the previous Fock matrix is kept across the switch. */
 int hfg_set_precision(hfg_ctx* ctx, int precision);
 
+/* Density threshold of the following incremental builds (hfg_options.density_tol; >= 0, 0 turns
+ * density screening off). No counterpart in hfscf, which has no incremental builds: the SCF
+ * driver tightens it for builds whose errors must stay below the energy tolerance. */
+int hfg_set_density_tol(hfg_ctx* ctx, double density_tol);
+
 /* E_2e = sum_ij D_ij (2J - K)_ij for the half density D. */
 int hfg_energy(hfg_ctx* ctx, const double* D, double* E2);
 

@@ -607,26 +607,29 @@ void orc_build_eri(void* h) {
 }
 
 // Build ERIs only for the shell-quartet rows i in [i0, i1) (bounded CPU timing sample);
-// returns the number of shell quartets visited.
+// returns the number of shell quartets visited. The (i, j) pairs of the rows are dealt to the
+// threads, so that a sample of one row uses every thread as the full build does.
 long long orc_build_eri_rows(void* h, int i0, int i1) {
     Oracle& o = *(Oracle*)h;
     if (o.eri.size() != (size_t)orc_num_eri(h)) o.eri.assign((size_t)orc_num_eri(h), 0.0);
     const int ns = o.nsh();
+    const long long npair = (long long)std::max(i1 - i0, 0) * ns;
     long long count = 0;
 #pragma omp parallel reduction(+ : count)
     {
         QuartetWork w;
 #pragma omp for schedule(dynamic)
-        for (int i = i0; i < i1; ++i)
-            for (int j = i; j < ns; ++j) {
-                const int64_t ij = index_ij(i, j);
-                for (int k = 0; k < ns; ++k)
-                    for (int l = k; l < ns; ++l) {
-                        if (ij > index_ij(k, l)) continue;
-                        quartet_store(o, i, j, k, l, w);
-                        ++count;
-                    }
-            }
+        for (long long p = 0; p < npair; ++p) {
+            const int i = i0 + (int)(p / ns), j = (int)(p % ns);
+            if (j < i) continue;
+            const int64_t ij = index_ij(i, j);
+            for (int k = 0; k < ns; ++k)
+                for (int l = k; l < ns; ++l) {
+                    if (ij > index_ij(k, l)) continue;
+                    quartet_store(o, i, j, k, l, w);
+                    ++count;
+                }
+        }
     }
     return count;
 }

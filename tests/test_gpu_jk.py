@@ -173,3 +173,21 @@ def test_cfg3_c12h26_fixture_rows():
           f"sum D.K {np.sum(D * K) - float(f['EK']):.2e}", flush=True)
     assert dj < 1e-11 and dk < 1e-11
     assert abs(np.sum(D * J) - float(f["EJ"])) < 1e-9 and abs(np.sum(D * K) - float(f["EK"])) < 1e-9
+
+
+@pytest.mark.parametrize("precision,incremental", [(0, 0), (0, 1), (1, 1)])
+def test_device_scf_step_matches_host_step(precision, incremental):
+    # The SCF step kept on the device (rocBLAS/rocSOLVER: orthogonaliser, DIIS, eigensolver,
+    # density, energy) against the host step of the same loop, both on the GPU Fock builds.
+    mol = _mol("h2o_8.xyz", "cc-pvdz")
+    opts = dict(diis_range=8, energy_tol=1e-10, rms_tol=1e-10, precision=precision, incremental=incremental)
+    dev = H.rhf(mol, device_step=1, **opts)
+    host = H.rhf(mol, device_step=0, **opts)
+    dE = dev["energy_final"] - host["energy_final"]
+    dD = np.abs(dev["D"] - host["D"]).max()
+    print(f"precision {precision} incremental {incremental}: dE {dE:.2e} |dD| {dD:.2e} "
+          f"iterations {dev['iterations']}/{host['iterations']}", flush=True)
+    assert dev["converged"] and host["converged"]
+    assert abs(dE) < 1e-9
+    assert dD < 1e-6
+    assert abs(dev["iterations"] - host["iterations"]) <= 2
