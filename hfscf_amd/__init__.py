@@ -28,7 +28,8 @@ FP64, MIXED = 0, 1
 
 class HfgOptions(C.Structure):
     _fields_ = [("precision", C.c_int), ("integral_tol", C.c_double), ("schwarz", C.c_int),
-                ("density_tol", C.c_double), ("graph", C.c_int), ("reserved", C.c_int * 5)]
+                ("density_tol", C.c_double), ("graph", C.c_int), ("work_split", C.c_int),
+                ("coop_min_l", C.c_int), ("reserved", C.c_int * 3)]
 
 
 class HfgStats(C.Structure):
@@ -358,13 +359,15 @@ class GpuContext:
     list (hfgpu C ABI; batched matrices are concatenated per molecule)."""
 
     def __init__(self, mol, precision=FP64, integral_tol=1e-14, schwarz=True, device=0, density_tol=0.0,
-                 graph=False):
+                 graph=False, work_split=0, coop_min_l=0):
         lib = hfgpu()
         opt = HfgOptions()
         lib.hfg_default_options(C.byref(opt))
         opt.precision, opt.integral_tol, opt.schwarz = precision, integral_tol, int(schwarz)
         opt.density_tol = density_tol
         opt.graph = int(graph)
+        opt.work_split = int(work_split)
+        opt.coop_min_l = int(coop_min_l)
         h = C.c_void_p()
         rc = lib.hfg_create(C.byref(h), device, C.byref(opt))
         if rc:

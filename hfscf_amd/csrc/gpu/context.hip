@@ -466,7 +466,8 @@ int setup_geometry(hfg_ctx* c) {
                     class_cost[ci] += cost;
                 }
             }
-    const double target = std::max(total_cost / (256.0 * 6.0), 1.0);
+    const double split = c->opt.work_split > 0 ? (double)c->opt.work_split : kDefaultWorkSplit;
+    const double target = std::max(total_cost / (256.0 * 6.0 * split), 1.0);
     c->slab_doubles = 0;
     c->grad_slab_doubles = 0;
     c->total_quartets = 0;
@@ -485,9 +486,8 @@ int setup_geometry(hfg_ctx* c) {
                 const int tile = c->blk_nf[A] * c->blk_nf[B] + c->blk_nf[C] * c->blk_nf[D] +
                                  c->blk_nf[A] * c->blk_nf[C] + c->blk_nf[A] * c->blk_nf[D] +
                                  c->blk_nf[B] * c->blk_nf[C] + c->blk_nf[B] * c->blk_nf[D];
-                // per-class item target: a class launch should have >= 256 items when it has the
-                // work (4x more made the gather 4x slower and the wide s and p classes no faster)
-                const double tgt = std::max(std::min(target, class_cost[ci] / 256.0), 1.0);
+                // per-class item target: a class launch has >= 256 x split items when it has the work
+                const double tgt = std::max(std::min(target, class_cost[ci] / (256.0 * split)), 1.0);
                 WorkItem w{};
                 w.P = P;
                 w.Q = Q;
@@ -558,13 +558,17 @@ int setup_geometry(hfg_ctx* c) {
         std::vector<ClassWork> sorted;
         for (size_t k : ord) sorted.push_back(c->classes[k]);
         c->classes.swap(sorted);
-        // programs of the classes on the cooperative J/K kernel (when their LDS fits a workgroup)
+        // cooperative programs of every class from L = kCoopTuneMinL (when their LDS fits a
+        // workgroup); the classes from coop_min_l up start on the cooperative kernels, autotuning
+        // may move the others there
+        const int coop_min = c->opt.coop_min_l > 0 ? c->opt.coop_min_l : kCoopRequiredL;
+        auto class_l = [](const ClassWork& cw) { return cw.la + cw.lb + cw.lc + cw.ld; };
         std::vector<int4> words;
         std::vector<std::pair<size_t, CoopHostProgram>> progs;
         for (size_t k = 0; k < c->classes.size(); ++k) {
             ClassWork& cw = c->classes[k];
-            cw.coop = false;
-            if (!jk_uses_coop(cw.la, cw.lb, cw.lc, cw.ld)) continue;
+            cw.coop = cw.coop_ok = false;
+            if (class_l(cw) < kCoopTuneMinL) continue;
             CoopHostProgram hp = make_coop_program(cw.la, cw.lb, cw.lc, cw.ld, c->pure);
             cw.coop_lds[0] = hp.lds_bytes(false);
             cw.coop_lds[1] = hp.lds_bytes(true);
@@ -572,15 +576,16 @@ int setup_geometry(hfg_ctx* c) {
             const size_t base = words.size();
             words.insert(words.end(), hp.words.begin(), hp.words.end());
             progs.emplace_back(base, std::move(hp));
-            cw.coop = true;
+            cw.coop_ok = true;
+            cw.coop = class_l(cw) >= coop_min || jk_uses_coop(cw.la, cw.lb, cw.lc, cw.ld);
         }
-        // gradient programs of the classes on the cooperative gradient kernel
+        // gradient programs of the same classes for the cooperative gradient kernel
         std::vector<int4> gwords;
         std::vector<std::pair<size_t, CoopGradHostProgram>> gprogs;
         for (size_t k = 0; k < c->classes.size(); ++k) {
             ClassWork& cw = c->classes[k];
-            cw.gcoop = false;
-            if (!grad_uses_coop(cw.la, cw.lb, cw.lc, cw.ld)) continue;
+            cw.gcoop = cw.gcoop_ok = false;
+            if (class_l(cw) < kCoopTuneMinL) continue;
             const size_t limit = (size_t)c->max_lds - kCoopGradStaticLds;
             CoopGradHostProgram hp;
             try {
@@ -593,13 +598,14 @@ int setup_geometry(hfg_ctx* c) {
             const size_t base = gwords.size();
             gwords.insert(gwords.end(), hp.words.begin(), hp.words.end());
             gprogs.emplace_back(base, std::move(hp));
-            cw.gcoop = true;
+            cw.gcoop_ok = true;
+            cw.gcoop = class_l(cw) >= coop_min || grad_uses_coop(cw.la, cw.lb, cw.lc, cw.ld);
         }
         if (!gwords.empty()) {
             HFG_CHECK(c, c->d_gcoop.upload(gwords, c->stream));
             size_t i = 0;
             for (size_t k = 0; k < c->classes.size(); ++k)
-                if (c->classes[k].gcoop) {
+                if (c->classes[k].gcoop_ok) {
                     c->classes[k].gcoop_args = coop_grad_args_on_device(gprogs[i].second, c->d_gcoop.p + gprogs[i].first);
                     ++i;
                 }
@@ -611,7 +617,7 @@ int setup_geometry(hfg_ctx* c) {
             HFG_CHECK(c, c->d_coop.upload(words, c->stream));
             size_t i = 0;
             for (size_t k = 0; k < c->classes.size(); ++k)
-                if (c->classes[k].coop) {
+                if (c->classes[k].coop_ok) {
                     c->classes[k].coop_args = coop_args_on_device(progs[i].second, c->d_coop.p + progs[i].first);
                     ++i;
                 }
@@ -1036,7 +1042,8 @@ int hfg_create(hfg_ctx** out, int device, const hfg_options* opt) {
         c->opt = *opt;
     else
         hfg_default_options(&c->opt);
-    if (c->opt.precision != HFG_FP64 && c->opt.precision != HFG_MIXED) {
+    if ((c->opt.precision != HFG_FP64 && c->opt.precision != HFG_MIXED) ||
+        (c->opt.coop_min_l != 0 && (c->opt.coop_min_l < kCoopTuneMinL || c->opt.coop_min_l > kCoopRequiredL))) {
         delete c;
         return HFG_ERR_ARG;
     }
@@ -1269,33 +1276,50 @@ int hfg_autotune(hfg_ctx* c, int what) {
     int r = check_ready(c);
     if (r) return r;
     if (what < 1 || what > 3) return fail(c, HFG_ERR_ARG, "hfg_autotune: what must be 1, 2 or 3");
-    // candidate shapes: threads per workgroup x persistent workgroups (0 = one per work item)
+    // candidates: the register kernel at threads per workgroup x persistent workgroups (0 = one
+    // per work item), and for the classes that have a cooperative program that kernel at each
+    // grid (its workgroup size is fixed)
     const int blocks[2] = {256, 128};
     const int grids[3] = {0, 512, 2048};
     destroy_graphs(c);
     for (int kind = 1; kind <= 2; ++kind) {
         if (!(what & kind)) continue;
+        const bool grad = kind == 2;
         std::vector<double> measured(c->classes.size(), 0.0);
         for (size_t k = 0; k < c->classes.size(); ++k) {
             ClassWork& cw = c->classes[k];
-            LaunchShape best = kind == 1 ? cw.jk_shape : cw.grad_shape;
+            bool& use_coop = grad ? cw.gcoop : cw.coop;
+            const bool coop_ok = grad ? cw.gcoop_ok : cw.coop_ok;
+            const bool reg_ok = cw.la + cw.lb + cw.lc + cw.ld < kCoopRequiredL;
+            LaunchShape& shape = grad ? cw.grad_shape : cw.jk_shape;
+            bool best_coop = use_coop;
+            LaunchShape best = shape;
             float best_ms = 1e30f;
-            for (int b : blocks)
-                for (int g : grids) {
-                    const LaunchShape sh{g, b};
-                    float total = 0.0f;
-                    for (int rep = 0; rep < 2; ++rep) {
-                        if ((r = hfg::run_one_class(c, k, kind == 2, sh))) return r;
-                        float ms = 0.0f;
-                        HFG_CHECK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
-                        if (rep > 0) total += ms;  // the first run warms caches and code
-                    }
-                    if (total < best_ms) {
-                        best_ms = total;
-                        best = sh;
-                    }
+            auto trial = [&](bool coop, LaunchShape sh) -> int {
+                use_coop = coop;
+                float total = 0.0f;
+                for (int rep = 0; rep < 2; ++rep) {
+                    if (int e = hfg::run_one_class(c, k, grad, sh)) return e;
+                    float ms = 0.0f;
+                    HFG_CHECK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+                    if (rep > 0) total += ms;  // the first run warms caches and code
                 }
-            (kind == 1 ? cw.jk_shape : cw.grad_shape) = best;
+                if (total < best_ms) {
+                    best_ms = total;
+                    best = sh;
+                    best_coop = coop;
+                }
+                return HFG_OK;
+            };
+            if (reg_ok)
+                for (int b : blocks)
+                    for (int g : grids)
+                        if ((r = trial(false, LaunchShape{g, b}))) return r;
+            if (coop_ok)
+                for (int g : grids)
+                    if ((r = trial(true, LaunchShape{g, 0}))) return r;
+            use_coop = best_coop;
+            shape = best;
             measured[k] = best_ms;
         }
         // rebalance the side streams on the measured times of the chosen shapes

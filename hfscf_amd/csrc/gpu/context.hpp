@@ -10,6 +10,7 @@
 #include "../../../include/hfgpu.h"
 #include "coop_args.hpp"
 #include "kernels.hpp"
+#include "launch_bounds.hpp"
 
 namespace hfg {
 
@@ -39,6 +40,10 @@ struct DevBuf {
 };
 
 constexpr int kSideStreams = 4;  // class launches run concurrently on these
+// Classes from L = la+lb+lc+ld = kCoopRequiredL have only the cooperative kernels; from
+// kCoopTuneMinL up both forms exist and hfg_options.coop_min_l / hfg_autotune choose.
+constexpr int kCoopRequiredL = 7;
+constexpr int kCoopTuneMinL = 4;
 
 inline int nfun(int L, bool pure) { return pure ? 2 * L + 1 : (L + 1) * (L + 2) / 2; }
 
@@ -56,10 +61,12 @@ struct ClassWork {
     // J/K on the wavefront-cooperative kernel: its program (device pointers into the context's
     // d_coop buffer) and the LDS bytes of its FP64 and mixed launches
     bool coop = false;
+    bool coop_ok = false;  // a program exists (the class may move between the two kernel forms)
     CoopArgs coop_args{};
     size_t coop_lds[2] = {0, 0};
     // the gradient on the cooperative kernel (program in d_gcoop)
     bool gcoop = false;
+    bool gcoop_ok = false;
     CoopGradArgs gcoop_args{};
     size_t gcoop_lds[2] = {0, 0};
 };
@@ -75,6 +82,12 @@ struct MolHost {
     int dm_off = 0;              // offset of its nshell x nshell density-maximum table
     int gout0 = 0;               // offset of its natom x 3 gradient in concatenated output
 };
+
+// Default split of a class launch into work items (hfg_options.work_split replaces it): four times
+// the items of one per 1/1536 of the build's model cost, which keeps several wavefronts per SIMD
+// busy in the wide s and p classes (measured on (H2O)16 / cc-pVDZ: 48 ms per FP64 build at 1,
+// 37 at 2, 36 at 4, 38 at 8).
+constexpr double kDefaultWorkSplit = 4.0;
 
 // Cached graph of one J/K build configuration.
 struct JkGraph {
@@ -237,10 +250,14 @@ inline hipError_t for_each_class_concurrent(hfg_ctx* c, hipStream_t main, F&& fn
     return join_shared ? hipStreamWaitEvent(main, c->ev_heavy_join, 0) : hipSuccess;
 }
 
-// Stream of a class launch (J/K or gradient).
+// Stream of a class launch (J/K or gradient): a register kernel that spills always runs on the
+// device's shared stream, also while autotuning tries it for a class that is on a side stream.
 inline hipStream_t class_launch_stream(hfg_ctx* c, size_t k, bool grad) {
+    const ClassWork& cw = c->classes[k];
     const int s = (grad ? c->grad_stream : c->class_stream)[k];
-    return s < 0 ? c->heavy : c->side[s];
+    const bool coop = grad ? cw.gcoop : cw.coop;
+    if (s < 0 || (!coop && class_spills(cw.la, cw.lb, cw.lc, cw.ld, grad))) return c->heavy;
+    return c->side[s];
 }
 
 // The J/K launch of one class (the cooperative kernel or the class kernel) on stream st.

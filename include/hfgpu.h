@@ -52,7 +52,14 @@ typedef struct hfg_options {
                              |dD_bc|, |dD_bd|) is below this (the bound of hfscf's direct digest,
                              hfscf_hf_solver.cpp:317-324); 0 = off (default) */
     int graph;            /* 1 = capture each J/K build configuration once as a HIP graph and replay it */
-    int reserved[5];
+    int work_split;       /* work items per class launch relative to the default split (1, 2, 4, ...;
+                             0 = the default). More items keep more wavefronts in flight at the cost of
+                             more tiles to gather. */
+    int coop_min_l;       /* classes with la+lb+lc+ld >= coop_min_l run on the wavefront-cooperative
+                             kernels (one workgroup per quartet, recursion tables in LDS), the others
+                             on the per-lane register kernels; 4..7, 0 = 7 (the default). From L = 4 up,
+                             hfg_autotune keeps whichever form it measures faster for each class. */
+    int reserved[3];
 } hfg_options;
 
 /* Fill opt with defaults (FP64, integral_tol 1e-14, screening on, no density screening). */

@@ -37,6 +37,22 @@ def _quartets(L):
         for b in d:
             qs += [(a, b, d[0], d[-1]), (a, b, d[-1], p[0]), (a, p[1], d[0], d[-1]), (a, p[0], b, p[-1]),
                    (a, b, d[0], 0), (a, b, p[2], p[1])]
+    return qs + _low_classes(L)
+
+
+def _low_classes(L):
+    # one quartet of every class from L = 4 up to 6 (la >= lb, lc >= ld), on shells of different
+    # atoms where the basis has them: the classes the library may also run on these kernels
+    by_l = {l: [i for i in range(len(L)) if L[i] == l] for l in range(3)}
+    qs = []
+    for la in range(3):
+        for lb in range(la + 1):
+            for lc in range(3):
+                for ld in range(lc + 1):
+                    if not 4 <= la + lb + lc + ld <= 6:
+                        continue
+                    pick = [by_l[l][(k * 3 + 1) % len(by_l[l])] for k, l in enumerate((la, lb, lc, ld))]
+                    qs.append(tuple(pick))
     return qs
 
 
@@ -63,6 +79,7 @@ def test_coop_programs_against_oracle(evaluator, basis, pure):
         assert 0 < lds <= 64 * 1024 - 2048  # FP64 LDS of the launch fits a 64 KB workgroup
         worst = max(worst, float(np.abs(out.reshape(nf) - o.shell_quartet(*q)).max()))
     assert worst < 1e-14, worst
+    assert len(_low_classes(L)) == 22 - 3  # every class from L = 4 to 6
 
 
 def test_coop_gradient_programs_against_oracle(evaluator):
@@ -78,7 +95,7 @@ def test_coop_gradient_programs_against_oracle(evaluator):
     d = [i for i in range(len(L)) if L[i] == 2]
     p = [i for i in range(len(L)) if L[i] == 1]
     qs = [(d[0], d[1], d[1], d[0]), (d[0], d[1], d[0], p[0]), (d[1], p[1], d[0], d[1]), (d[0], d[0], d[1], p[2]),
-          (d[1], p[0], d[0], p[3]), (d[0], d[1], p[2], p[1]), (d[0], 0, d[1], d[1])]
+          (d[1], p[0], d[0], p[3]), (d[0], d[1], p[2], p[1]), (d[0], 0, d[1], d[1])] + _low_classes(L)
     dp, ip = C.POINTER(C.c_double), C.POINTER(C.c_int)
     rng = np.random.default_rng(0)
     for q in qs:

@@ -33,6 +33,18 @@ def test_grad2e_water_fixture(basis):
     assert dt < 1e-8
 
 
+@pytest.mark.parametrize("basis", ["cc-pvdz", "6-31g*"])
+def test_grad2e_cooperative_kernels_from_l4(basis):
+    # the gradient classes from L = 4 on the cooperative gradient kernel, against the oracle
+    f = np.load(golden("grad_h2o.npz"))
+    key = basis.replace("*", "s").replace("-", "")
+    mol = H.Molecule.from_xyz(golden("h2o.xyz"), basis)
+    g = H.GpuContext(mol, coop_min_l=4).energy_backward(f[f"{key}_D"])
+    d = np.abs(g - f[f"{key}_g2e"]).max()
+    print(f"{basis}, coop from L=4: max |g2e - oracle| = {d:.2e}", flush=True)
+    assert d < 1e-10
+
+
 def test_grad2e_dimer_live_oracle():
     from oracle import basis_ref
     Z, X = basis_ref.read_xyz(golden("h2o_8.xyz"))
@@ -41,11 +53,12 @@ def test_grad2e_dimer_live_oracle():
     A = rng.standard_normal((mol.nbf, mol.nbf))
     D = 0.05 * (A + A.T)
     g = H.GpuContext(mol).energy_backward(D)
+    gc = H.GpuContext(mol, coop_min_l=4).energy_backward(D)
     orc = oracle.Oracle(mol.shells(), mol.geometry()[1], mol.pure, 1e-14, screen=False)
     gr = orc.grad2e(D)
-    d = np.abs(g - gr).max()
-    print(f"(H2O)2 cc-pVDZ random D: max |g2e - oracle| = {d:.2e}", flush=True)
-    assert d < 1e-9
+    d, dc = np.abs(g - gr).max(), np.abs(gc - gr).max()
+    print(f"(H2O)2 cc-pVDZ random D: max |g2e - oracle| = {d:.2e} (cooperative from L=4: {dc:.2e})", flush=True)
+    assert d < 1e-9 and dc < 1e-9
 
 
 def _energy(mol, xyz, **opts):

@@ -191,3 +191,21 @@ def test_device_scf_step_matches_host_step(precision, incremental):
     assert abs(dE) < 1e-9
     assert dD < 1e-6
     assert abs(dev["iterations"] - host["iterations"]) <= 2
+
+
+@pytest.mark.parametrize("basis,dimer", [("cc-pvdz", True), ("6-31g*", False), ("cc-pvdz", False)])
+def test_jk_cooperative_kernels_from_l4(basis, dimer):
+    # every class from L = 4 on the wavefront-cooperative kernel instead of the per-lane register
+    # kernels (the choice autotuning may make per class), against the oracle
+    mol = H.Molecule.from_arrays(*_dimer(), basis) if dimer else _mol("h2o.xyz", basis)
+    ctx = H.GpuContext(mol, coop_min_l=4)
+    orc = oracle.Oracle(mol.shells(), mol.geometry()[1], mol.pure, 1e-14, screen=True)
+    D = _rand_density(mol.nbf, 5)
+    J, K = ctx.build_jk(D)
+    Jr, Kr = orc.jk(D)
+    J4, K4 = H.GpuContext(mol).build_jk(D)
+    print(f"{basis} dimer={dimer}, coop from L=4: |dJ| {np.abs(J - Jr).max():.2e} |dK| {np.abs(K - Kr).max():.2e}; "
+          f"vs register kernels |dJ| {np.abs(J - J4).max():.2e} |dK| {np.abs(K - K4).max():.2e}", flush=True)
+    assert np.abs(J - Jr).max() < 1e-11 and np.abs(K - Kr).max() < 1e-11
+    with pytest.raises(RuntimeError):
+        H.GpuContext(mol, coop_min_l=3)
