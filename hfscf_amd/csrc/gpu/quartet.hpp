@@ -1,6 +1,9 @@
 // Contracted shell-quartet evaluation and J/K digestion for one (la lb|lc ld) class.
 #pragma once
+#include <type_traits>
+
 #include "boys.hpp"
+#include "boys32.hpp"
 #include "hgp.hpp"
 #include "layout.hpp"
 
@@ -90,6 +93,62 @@ HFG_INLINE void eri_cart(const PairDev& bp, const PairDev& kp, const double4* __
     using H = Hrr<LA, LB, LC, LD, R>;
     R acc[H::NE][H::NF];
     sfor<H::NE>([&](auto e) { sfor<H::NF>([&](auto f) { acc[e][f] = (R)0; }); });
+    if constexpr (std::is_same_v<R, float> && LT <= 2) {
+        // mixed precision, L <= 2: the whole primitive quartet in FP32 (the centre differences
+        // P - Q in FP64 first), Boys values from boys32
+        const int kn = kp.np, ks = kp.pstride;
+        const float cut = (float)prim_cut;
+        for (int i = 0; i < bp.np; ++i) {
+            const int ia = bp.off + i * bp.pstride;
+            const double4 a0 = pp0[ia];
+            const double4 a1 = pp1[ia];
+            const float o2p = (LT > 0) ? (float)pp2[ia].x : 0.0f;
+            const float p = (float)a0.x, Ka = (float)a1.w;
+            const float PA[3] = {(float)a1.x, (float)a1.y, (float)a1.z};
+            int ib = kp.off;
+            double4 b0 = pp0[ib], b1 = pp1[ib];
+            double o2q = (LT > 0) ? pp2[ib].x : 0.0;
+            for (int j = 0; j < kn; ++j) {
+                const int ibn = (j + 1 < kn) ? ib + ks : ib;  // stays in bounds on the last primitive
+                const double4 nb0 = pp0[ibn], nb1 = pp1[ibn];
+                const double no2q = (LT > 0) ? pp2[ibn].x : 0.0;
+                const float q = (float)b0.x;
+                const float rs = boys32_rsqrt(p + q);
+                float pfac = Ka * (float)b1.w * rs;
+                pfac = (fabsf(pfac) < cut) ? 0.0f : pfac;
+                const float PQx = (float)(a0.y - b0.y), PQy = (float)(a0.z - b0.z), PQz = (float)(a0.w - b0.w);
+                const float r2 = PQx * PQx + PQy * PQy + PQz * PQz;
+                const float ipq = rs * rs;
+                float T = p * q * ipq * r2;
+                float F[LT + 1];
+                if constexpr (LT == 0) {
+                    // the floor of the FP64 path (F_0 at T >= 1e-8, 1 when r_PQ^2 <= 1e-18)
+                    T = (r2 > 1e-18f) ? fmaxf(T, 1e-8f) : 0.0f;
+                    boys32<0>(T, pfac, F);
+                    acc[0][0] += F[0];
+                } else {
+                    boys32<LT>(T, pfac, F);
+                    const float rp = q * ipq, rq = p * ipq;
+                    const float QC[3] = {(float)b1.x, (float)b1.y, (float)b1.z};
+                    const float WP[3] = {-rp * PQx, -rp * PQy, -rp * PQz};
+                    const float WQ[3] = {rq * PQx, rq * PQy, rq * PQz};
+                    Vrr<LA + LB, LC + LD, float> V;
+                    V.run(F, PA, WP, QC, WQ, o2p, rp, (float)o2q, rq, 0.5f * ipq);
+                    sfor<H::NE>([&](auto e) {
+                        sfor<H::NF>([&](auto f) { acc[e][f] += V.v[H::NE0 + e][H::NF0 + f][0]; });
+                    });
+                }
+                b0 = nb0;
+                b1 = nb1;
+                o2q = no2q;
+                ib = ibn;
+            }
+        }
+        const R AB[3] = {(R)bp.AB[0], (R)bp.AB[1], (R)bp.AB[2]};
+        const R CD[3] = {(R)kp.AB[0], (R)kp.AB[1], (R)kp.AB[2]};
+        H::run(acc, AB, CD, out);
+        return;
+    }
     const int kn = kp.np, ks = kp.pstride;
     for (int i = 0; i < bp.np; ++i) {
         const int ia = bp.off + i * bp.pstride;
