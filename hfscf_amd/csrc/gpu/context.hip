@@ -673,6 +673,24 @@ int setup_geometry(hfg_ctx* c) {
         c->gJ_off.push_back((int)c->gJ.size());
         c->gK_off.push_back((int)c->gK.size());
     }
+    // first-pass chunks of kGatherChunk entries: the J chunks of every tile, then the K chunks
+    c->gchunks.clear();
+    c->cJ_off.assign(1, 0);
+    c->cK_off.clear();
+    c->gpart_doubles = 0;
+    for (int list = 0; list < 2; ++list) {
+        std::vector<int>& coff = list ? c->cK_off : c->cJ_off;
+        if (list) coff.push_back((int)c->gchunks.size());
+        const std::vector<int>& goff = list ? c->gK_off : c->gJ_off;
+        for (int t = 0; t < nbp; ++t) {
+            const int elems = c->blk_nf[c->bp_X[t]] * c->blk_nf[c->bp_Y[t]];
+            for (int g0 = goff[t]; g0 < goff[t + 1]; g0 += kGatherChunk) {
+                c->gchunks.push_back(GatherChunk{t, list, g0, std::min(g0 + kGatherChunk, goff[t + 1]), c->gpart_doubles});
+                c->gpart_doubles += elems;
+            }
+            coff.push_back((int)c->gchunks.size());
+        }
+    }
     std::vector<int> mol_items, mol_items_off(1, 0), out_mol, out_k;
     for (size_t m = 0; m < c->mols.size(); ++m) {
         mol_items.insert(mol_items.end(), mitems[m].begin(), mitems[m].end());
@@ -700,6 +718,10 @@ int setup_geometry(hfg_ctx* c) {
     HFG_CHECK(c, c->d_gK_off.upload(c->gK_off, c->stream));
     HFG_CHECK(c, c->d_gJ.upload(c->gJ, c->stream));
     HFG_CHECK(c, c->d_gK.upload(c->gK, c->stream));
+    HFG_CHECK(c, c->d_gchunks.upload(c->gchunks, c->stream));
+    HFG_CHECK(c, c->d_cJ_off.upload(c->cJ_off, c->stream));
+    HFG_CHECK(c, c->d_cK_off.upload(c->cK_off, c->stream));
+    HFG_CHECK(c, c->d_gpart.ensure((size_t)std::max<long long>(c->gpart_doubles, 1)));
     HFG_CHECK(c, c->d_shell_atom.upload(c->atom_local, c->stream));
     {
         std::vector<int> ao(c->nshell), nf(c->nshell);
@@ -762,16 +784,20 @@ int upload_dmax(hfg_ctx* c, const double* D) {
 }
 
 // Device buffers one J/K build reads and writes: the density, J and K, the work items' tile
-// slab and the class launches' queue counters (the context's own, or caller-provided ones).
+// slab, the class launches' queue counters and the gather's chunk sums (the context's own, or
+// caller-provided ones).
 struct JkBuffers {
     const double* D;
     double* J;
     double* K;
     double* slab;
     int* queue;
+    double* part;  // the gather's chunk sums
 };
 
-JkBuffers own_buffers(hfg_ctx* c) { return JkBuffers{c->d_D.p, c->d_J.p, c->d_K.p, c->d_slab.p, c->d_queue.p}; }
+JkBuffers own_buffers(hfg_ctx* c) {
+    return JkBuffers{c->d_D.p, c->d_J.p, c->d_K.p, c->d_slab.p, c->d_queue.p, c->d_gpart.p};
+}
 
 // The class launches of the set `set` on `main` (forked to the class streams and joined back),
 // each preceded by the reset of its queue counter: the shared-stream classes' counters
@@ -838,7 +864,8 @@ int enqueue_gather(hfg_ctx* c, hipStream_t main, const JkBuffers& buf) {
     g.mols = c->d_mols.p;
     g.J = buf.J;
     g.K = buf.K;
-    HFG_CHECK(c, launch_gather(g, (int)c->tiles.size(), main));
+    HFG_CHECK(c, launch_gather_chunks(g, c->d_gchunks.p, c->d_cJ_off.p, c->d_cK_off.p, (int)c->gchunks.size(),
+                                      (int)c->tiles.size(), buf.part, main));
     return HFG_OK;
 }
 
@@ -1173,7 +1200,7 @@ int hfg_build_jk_noalloc(hfg_ctx* c, const double* D, double* J, double* K) {
     if (r) return r;
     if (!D || !J || !K) return fail(c, HFG_ERR_ARG, "hfg_build_jk_noalloc: NULL argument");
     const size_t ncls = c->classes.size();
-    const size_t need = (size_t)c->slab_doubles;
+    const size_t need = (size_t)c->slab_doubles + (size_t)c->gpart_doubles;
     if (ncls > 2 * kNoallocCounterDoubles || need > kNoallocScratchDoubles - kNoallocCounterDoubles) {
         char msg[200];
         std::snprintf(msg, sizeof msg,
@@ -1184,7 +1211,8 @@ int hfg_build_jk_noalloc(hfg_ctx* c, const double* D, double* J, double* K) {
     }
     if ((r = check_private_segments(c, false))) return r;
     auto t0 = std::chrono::steady_clock::now();
-    const JkBuffers buf{D, J, K, c->scratch + kNoallocCounterDoubles, reinterpret_cast<int*>(c->scratch)};
+    double* const slab = c->scratch + kNoallocCounterDoubles;
+    const JkBuffers buf{D, J, K, slab, reinterpret_cast<int*>(c->scratch), slab + c->slab_doubles};
     HFG_CHECK(c, hipEventRecord(c->ev0, c->stream));
     if ((r = enqueue_classes(c, c->stream, false, false, ClassSet::All, buf))) return r;
     HFG_CHECK(c, hipEventRecord(c->ev1, c->stream));
@@ -1521,6 +1549,10 @@ void hfg_destroy(hfg_ctx* c) {
     c->d_blk_nf.release();
     c->d_blk_func_off.release();
     c->d_blk_func_ao.release();
+    c->d_gchunks.release();
+    c->d_cJ_off.release();
+    c->d_cK_off.release();
+    c->d_gpart.release();
     c->d_gJ_off.release();
     c->d_gK_off.release();
     c->d_shell_atom.release();

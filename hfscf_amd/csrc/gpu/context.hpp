@@ -9,6 +9,7 @@
 
 #include "../../../include/hfgpu.h"
 #include "coop_args.hpp"
+#include "gather.hpp"
 #include "kernels.hpp"
 #include "launch_bounds.hpp"
 
@@ -107,6 +108,7 @@ struct hfg_ctx {
     using PairDev = hfg::PairDev;
     using WorkItem = hfg::WorkItem;
     using GatherEntry = hfg::GatherEntry;
+    using GatherChunk = hfg::GatherChunk;
     int device = 0;
     int max_lds = 65536;  // LDS bytes one workgroup may use on the device
     double* scratch = nullptr;  // the device's fixed scratch of hfg_build_jk_noalloc
@@ -153,6 +155,9 @@ struct hfg_ctx {
     std::vector<int4> tiles;
     std::vector<int> gJ_off, gK_off;
     std::vector<GatherEntry> gJ, gK;
+    std::vector<GatherChunk> gchunks;  // first-pass chunks: every tile's J chunks, then its K chunks
+    std::vector<int> cJ_off, cK_off;    // per tile, its J / K chunks in gchunks
+    long long gpart_doubles = 0;        // chunk sums of one build
     long long slab_doubles = 0, grad_slab_doubles = 0, total_quartets = 0, total_items = 0;
     double total_flops = 0, total_grad_flops = 0;
     // device
@@ -170,6 +175,9 @@ struct hfg_ctx {
     DevBuf<WorkItem> d_items;
     DevBuf<int4> d_tiles;
     DevBuf<GatherEntry> d_gJ, d_gK;
+    DevBuf<GatherChunk> d_gchunks;
+    DevBuf<int> d_cJ_off, d_cK_off;
+    DevBuf<double> d_gpart;
     // launch graphs of the J/K build (keyed by precision and density screening)
     std::vector<hfg::JkGraph> graphs;
     bool class_timed = false;  // the last J/K build bracketed each class launch with events

@@ -67,11 +67,11 @@ struct LaunchShape {
 };
 
 // Launchers (one specialised kernel per class; la >= lb, la >= lc >= ld). mixed != 0 selects
-// the FP32 evaluation and digestion of the class (FP64 Boys values, tiles and sums).
+// the FP32 evaluation and digestion of the class (FP64 Boys values and prefactors except in the
+// classes with L <= 2, FP64 tiles and sums).
 hipError_t launch_jk(int la, int lb, int lc, int ld, bool pure, int mixed, const JkArgs& a, int nitems,
                      LaunchShape sh, hipStream_t s);
 hipError_t launch_screen(int la, int lb, const ScreenArgs& a, hipStream_t s);
-hipError_t launch_gather(const GatherArgs& a, int ntiles, hipStream_t s);
 // Sets n ints to zero (a kernel, so it may sit in a captured graph).
 hipError_t launch_zero_ints(int* p, int n, hipStream_t s);
 hipError_t launch_fock(const double* base, const double* J, const double* K, double* F, double* Facc, int nn,

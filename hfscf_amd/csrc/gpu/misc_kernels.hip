@@ -61,24 +61,39 @@ __device__ __forceinline__ double gather_sum(const GatherEntry* __restrict__ lis
     return (s[0] + s[1]) + (s[2] + s[3]);
 }
 
-// One workgroup per output tile (X >= Y): sums every work-item tile that targets it
-// (transposed where the work item held it as (Y, X)) in a fixed order, then writes the
-// symmetric J and K elements.
-__global__ __launch_bounds__(256) void gather_kernel(GatherArgs a) {
-    const int4 xy = a.tiles[blockIdx.x];
+// First pass: one workgroup per chunk, one thread per tile element, the chunk's entries summed
+// in list order.
+__global__ __launch_bounds__(256) void gather_part_kernel(GatherArgs a, const GatherChunk* __restrict__ chunks,
+                                                          double* __restrict__ part) {
+    const GatherChunk ch = chunks[blockIdx.x];
+    const int4 xy = a.tiles[ch.tile];
+    const int fX = a.blk_nf[xy.x], fY = a.blk_nf[xy.y];
+    const GatherEntry* list = ch.list ? a.gK : a.gJ;
+    for (int e = threadIdx.x; e < fX * fY; e += blockDim.x) {
+        const int u = e / fY, v = e - u * fY;
+        part[ch.part + e] = gather_sum(list, ch.g0, ch.g1, a.slab, u, v);
+    }
+}
+
+// Second pass: one workgroup per output tile, the chunk sums of its lists in chunk order.
+__global__ __launch_bounds__(256) void gather_final_kernel(GatherArgs a, const GatherChunk* __restrict__ chunks,
+                                                           const int* __restrict__ cJ_off,
+                                                           const int* __restrict__ cK_off,
+                                                           const double* __restrict__ part) {
+    const int t = blockIdx.x;
+    const int4 xy = a.tiles[t];
     const MolDev md = a.mols[xy.z];
     double* __restrict__ J = a.J + md.moff;
     double* __restrict__ K = a.K + md.moff;
     const int fX = a.blk_nf[xy.x], fY = a.blk_nf[xy.y];
     const int* aoX = a.blk_func_ao + a.blk_func_off[xy.x];
     const int* aoY = a.blk_func_ao + a.blk_func_off[xy.y];
-    const int j0 = a.gJ_off[blockIdx.x], j1 = a.gJ_off[blockIdx.x + 1];
-    const int k0 = a.gK_off[blockIdx.x], k1 = a.gK_off[blockIdx.x + 1];
     for (int e = threadIdx.x; e < fX * fY; e += blockDim.x) {
         const int u = e / fY, v = e - u * fY;
         if (xy.x == xy.y && u < v) continue;  // diagonal tile: one thread per symmetric element pair
-        const double sj = gather_sum(a.gJ, j0, j1, a.slab, u, v);
-        const double sk = gather_sum(a.gK, k0, k1, a.slab, u, v);
+        double sj = 0.0, sk = 0.0;
+        for (int k = cJ_off[t]; k < cJ_off[t + 1]; ++k) sj += part[chunks[k].part + e];
+        for (int k = cK_off[t]; k < cK_off[t + 1]; ++k) sk += part[chunks[k].part + e];
         const size_t i1 = (size_t)aoX[u] * md.n + aoY[v];
         const size_t i2 = (size_t)aoY[v] * md.n + aoX[u];
         J[i1] = sj;
@@ -88,9 +103,11 @@ __global__ __launch_bounds__(256) void gather_kernel(GatherArgs a) {
     }
 }
 
-hipError_t launch_gather(const GatherArgs& a, int ntiles, hipStream_t s) {
+hipError_t launch_gather_chunks(const GatherArgs& a, const GatherChunk* chunks, const int* cJ_off, const int* cK_off,
+                                int nchunks, int ntiles, double* part, hipStream_t s) {
     if (ntiles == 0) return hipSuccess;
-    hipLaunchKernelGGL(gather_kernel, dim3(ntiles), dim3(256), 0, s, a);
+    if (nchunks > 0) hipLaunchKernelGGL(gather_part_kernel, dim3(nchunks), dim3(256), 0, s, a, chunks, part);
+    hipLaunchKernelGGL(gather_final_kernel, dim3(ntiles), dim3(256), 0, s, a, chunks, cJ_off, cK_off, part);
     return hipGetLastError();
 }
 
