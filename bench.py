@@ -175,11 +175,12 @@ def dominant_roofline(classes, workload):
 
 
 def gradient_figures(mol, D, steps, warmup, autotune=True):
-    """FP64 and mixed two-electron gradients/s (adjoint kernel) at the workload's density."""
+    """FP64 and mixed two-electron gradients/s (adjoint kernel) at the workload's density, over
+    every shell quartet as hfscf's gradient (no Schwarz screen)."""
     out = {}
     res = {}
     for tag, prec in [("fp64", H.FP64), ("mixed", H.MIXED)]:
-        ctx = H.GpuContext(mol, precision=prec)
+        ctx = H.GpuContext(mol, precision=prec, schwarz=False)
         if autotune:
             ctx.autotune(2)
         el, kern, g = time_grads(ctx, D, steps, warmup)

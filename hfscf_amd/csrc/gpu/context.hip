@@ -325,17 +325,36 @@ int setup_geometry(hfg_ctx* c) {
                     }
             }
     }
-    // mean Schwarz value per pair (hfscf's shell skip compares the mean of sqrt|S_pq S_rs|,
-    // which factorises into a product of per-pair means; hfscf_oseri.cpp:36-46)
-    c->tol = c->opt.schwarz ? 100.0 * c->opt.integral_tol : 0.0;
+    // Sort key s_ab of every pair and the threshold on products s_ab s_cd below which a quartet is
+    // skipped. With screening: the mean Schwarz value (hfscf's shell skip compares the mean of
+    // sqrt|S_pq S_rs|, which factorises into a product of per-pair means; hfscf_oseri.cpp:36-46)
+    // against 100 x integral_tol. Without (hfscf's eri_screen off, and its gradient, which visits
+    // every quartet): a bound on the primitive prefactors against the primitive cutoff. A
+    // primitive quartet is skipped when |K_ab K_cd| / sqrt(p + q) < integral_tol / 1e4
+    // (hfscf_oseri.cpp:73,98-101, hfscf_oserideriv1.cpp:107,133-136), and
+    // |K_ab K_cd| / sqrt(p + q) <= s_ab s_cd with s_ab = max|K_ab| / (2 min p)^(1/4) over the pair's
+    // primitives (sqrt(p + q) >= sqrt(2) (p q)^(1/4)), so a quartet with s_ab s_cd below the cutoff
+    // has no primitive quartet left and is exactly zero.
+    c->tol = c->opt.schwarz ? 100.0 * c->opt.integral_tol : c->opt.integral_tol / 1e4;
+    c->elem_tol = c->opt.schwarz ? c->tol : 0.0;
     std::vector<std::vector<std::pair<double, const CandPair*>>> lists(nbp);
     std::vector<double> smax(c->mols.size(), 0.0);
     for (int g = 0; g < nbp; ++g)
         for (auto& cp : cand[g]) {
             const int na = fm_ncart(c->L[cp.ga]), nb = fm_ncart(c->L[cp.gb]);
             double s = 0;
-            for (int k = 0; k < na * nb; ++k) s += std::sqrt(std::fabs(diag[cp.diag_off + k]));
-            s /= (double)(na * nb);
+            if (c->opt.schwarz) {
+                for (int k = 0; k < na * nb; ++k) s += std::sqrt(std::fabs(diag[cp.diag_off + k]));
+                s /= (double)(na * nb);
+            } else {
+                double kb = 0.0, pmin = 1e300;
+                for (int k = 0; k < cp.p.np; ++k) {  // the pair's significant primitives (stored first)
+                    kb = std::max(kb, std::fabs(pp1[cp.p.off + k].w));
+                    pmin = std::min(pmin, pp0[cp.p.off + k].x);
+                }
+                // widened by 1e-12 against the rounding of the kernels' prefactor products
+                s = cp.p.np ? kb / std::pow(2.0 * pmin, 0.25) * (1.0 + 1e-12) : 0.0;
+            }
             const int m = c->shell_mol[cp.ga];
             smax[m] = std::max(smax[m], s);
             lists[g].push_back({s, &cp});
@@ -819,7 +838,7 @@ int enqueue_classes(hfg_ctx* c, hipStream_t main, bool dscreen, bool timed, Clas
     a.mols = c->d_mols.p;
     a.tol = c->tol;
     a.psq = c->d_psq.p;
-    a.elem_tol = c->tol;
+    a.elem_tol = c->elem_tol;
     a.prim_cut = c->opt.integral_tol / 1e4;
     a.slab = buf.slab;
     a.dmax = c->d_dmax.p;
@@ -1305,8 +1324,8 @@ int hfg_autotune(hfg_ctx* c, int what) {
     if (r) return r;
     if (what < 1 || what > 3) return fail(c, HFG_ERR_ARG, "hfg_autotune: what must be 1, 2 or 3");
     // candidates: the register kernel at threads per workgroup x persistent workgroups (0 = one
-    // per work item), and for the classes that have a cooperative program that kernel at each
-    // grid (its workgroup size is fixed)
+    // per work item), and for the classes from L = kCoopTrialMinL that have a cooperative program
+    // that kernel at each grid (its workgroup size is fixed; block 0 marks it in the shapes)
     const int blocks[2] = {256, 128};
     const int grids[3] = {0, 512, 2048};
     destroy_graphs(c);
@@ -1343,7 +1362,7 @@ int hfg_autotune(hfg_ctx* c, int what) {
                 for (int b : blocks)
                     for (int g : grids)
                         if ((r = trial(false, LaunchShape{g, b}))) return r;
-            if (coop_ok)
+            if (coop_ok && (!reg_ok || cw.la + cw.lb + cw.lc + cw.ld >= kCoopTrialMinL))
                 for (int g : grids)
                     if ((r = trial(true, LaunchShape{g, 0}))) return r;
             use_coop = best_coop;
@@ -1364,9 +1383,9 @@ int hfg_launch_shapes(const hfg_ctx* c, int cap, int* la_lb_lc_ld, int* jk_grid,
         const ClassWork& cw = c->classes[k];
         if (la_lb_lc_ld) la_lb_lc_ld[k] = cw.la * 1000 + cw.lb * 100 + cw.lc * 10 + cw.ld;
         if (jk_grid) jk_grid[k] = cw.jk_shape.grid;
-        if (jk_block) jk_block[k] = cw.jk_shape.block;
+        if (jk_block) jk_block[k] = cw.coop ? -1 : cw.jk_shape.block;
         if (grad_grid) grad_grid[k] = cw.grad_shape.grid;
-        if (grad_block) grad_block[k] = cw.grad_shape.block;
+        if (grad_block) grad_block[k] = cw.gcoop ? -1 : cw.grad_shape.block;
     }
     return n;
 }

@@ -45,6 +45,10 @@ constexpr int kSideStreams = 4;  // class launches run concurrently on these
 // kCoopTuneMinL up both forms exist and hfg_options.coop_min_l / hfg_autotune choose.
 constexpr int kCoopRequiredL = 7;
 constexpr int kCoopTuneMinL = 4;
+// Autotuning tries the cooperative form from this L only: below it the per-lane register kernels
+// won every measurement on (H2O)8 and (H2O)16 / cc-pVDZ (by up to 5x at L = 4), and the trials of
+// large classes cost seconds in a batch.
+constexpr int kCoopTrialMinL = 6;
 
 inline int nfun(int L, bool pure) { return pure ? 2 * L + 1 : (L + 1) * (L + 2) / 2; }
 
@@ -145,11 +149,12 @@ struct hfg_ctx {
     std::vector<int> bp_X, bp_Y;
     // pairs
     std::vector<PairDev> pairs;
-    std::vector<double> psmean;
+    std::vector<double> psmean;  // sort key of each pair (mean Schwarz value, or the primitive bound)
     std::vector<double> psq;  // sqrt|(pq|pq)| per pair component
     std::vector<int> bp_off, bp_cnt;
     std::vector<double> screen;  // Cartesian (pq|pq) of molecule 0 (single-molecule contexts)
-    double tol = 0.0;
+    double tol = 0.0;       // quartets with psmean_ab psmean_cd below this are skipped
+    double elem_tol = 0.0;  // hfscf's element screen and Cartesian diagonal (0 without screening)
     // work
     std::vector<ClassWork> classes;
     std::vector<int4> tiles;

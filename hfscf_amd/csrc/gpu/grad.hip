@@ -43,7 +43,7 @@ JkArgs jk_args(hfg_ctx* c) {
     a.mols = c->d_mols.p;
     a.tol = c->tol;
     a.psq = c->d_psq.p;
-    a.elem_tol = c->tol;
+    a.elem_tol = c->elem_tol;
     a.prim_cut = c->opt.integral_tol / 1e4;
     a.slab = c->d_slab.p;
     a.dmax = c->d_dmax.p;

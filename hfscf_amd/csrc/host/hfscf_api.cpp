@@ -309,7 +309,28 @@ int batch_rhf(int nmol, hfscf_mol* const* mols, const hfscf_scf_options& opt, hf
             natot += 3 * mols[m]->mol.natom();
         }
         std::vector<double> g2(natot);
-        check(hfg_energy_backward(ctx, D.data(), g2.data()));
+        // hfscf's gradient visits every shell quartet (hfscf_grad_ints.cpp:110-151): a second
+        // batch context without the Schwarz screen
+        hfg_ctx* gctx = ctx;
+        std::unique_ptr<hfg_ctx, void (*)(hfg_ctx*)> gguard(nullptr, hfg_destroy);
+        if (go.schwarz) {
+            hfg_options gu = go;
+            gu.schwarz = 0;
+            gctx = nullptr;
+            if ((r = hfg_create(&gctx, opt.device, &gu)))
+                throw std::runtime_error("hfg_create failed (" + std::to_string(r) + "): no usable GPU");
+            gguard.reset(gctx);
+            for (int m = 0; m < nmol; ++m) {
+                BasisArrays b(mols[m]->mol);
+                check(hfg_batch_add_molecule(gctx, (int)b.L.size(), b.L.data(), b.atom.data(), b.np.data(),
+                                             b.alpha.data(), b.coef.data(), mols[m]->mol.pure ? 1 : 0, b.co.data(),
+                                             b.po.data(), mols[m]->mol.natom(), b.xyz.data()));
+            }
+            check(hfg_batch_commit(gctx));
+            check(hfg_set_precision(gctx, opt.precision));
+        }
+        if (hfg_energy_backward(gctx, D.data(), g2.data()) < 0)
+            throw std::runtime_error(std::string("hfgpu: ") + hfg_last_error(gctx));
 #pragma omp parallel for schedule(dynamic)
         for (int m = 0; m < nmol; ++m) {
             const Molecule& mol = mols[m]->mol;
@@ -538,6 +559,9 @@ int hfscf_rhf_gradient(hfscf_mol* m, const hfscf_scf_options* o, const hfscf_cal
             opt = *o;
         else
             hfscf_default_scf_options(&opt);
+        // hfscf's gradient visits every shell quartet (hfscf_grad_ints.cpp:110-151): no Schwarz
+        // screen for the two-electron part
+        opt.schwarz = 0;
         auto jk = make_backend(m->mol, opt, cb);
         auto g = calc_scf_gradient_rhf(m->mol, D, F, jk.get());
         std::memcpy(grad, g.data(), g.size() * sizeof(double));

@@ -176,7 +176,8 @@ int hfg_batch_layout(const hfg_ctx* ctx, int* nmol, int* nbf, long long* moff, i
  * fastest per class. what: 1 = J/K, 2 = gradient, 3 = both. */
 int hfg_autotune(hfg_ctx* ctx, int what);
 /* Current launch shape per class (la*1000+lb*100+lc*10+ld, grid, block for J/K and gradient;
- * grid 0 = one workgroup per work item). Returns the number of classes. */
+ * grid 0 = one workgroup per work item, block 0 = the default, block -1 = the class runs on the
+ * cooperative kernel, whose workgroup size is fixed). Returns the number of classes. */
 int hfg_launch_shapes(const hfg_ctx* ctx, int cap, int* la_lb_lc_ld, int* jk_grid, int* jk_block, int* grad_grid,
                       int* grad_block);
 
