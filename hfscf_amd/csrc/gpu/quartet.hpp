@@ -56,10 +56,14 @@ template <int M, typename R>
 __device__ __forceinline__ void boys_low(double T, double scale, R* F) {
     static_assert(M <= 1, "closed form for F_0 and F_1 only");
     double f0 = 1.0, ri = 0.0;
-    if (T > 0.0) {
-        ri = rsqrt(T);
+    if (T > 0.0) ri = rsqrt(T);
+    // erf(sqrt T) = 1 to within erfc(6) < 2.2e-17 from T = 36 on: the lanes there take F_0 =
+    // sqrt(pi/T)/2 directly, and a wavefront whose lanes all have T >= 36 (tight primitives on
+    // distant centres) skips the erf evaluation
+    if (T >= kBoysTmax)
+        f0 = 0.88622692545275801365 * ri;
+    else if (T > 0.0)
         f0 = 0.88622692545275801365 * erf(T * ri) * ri;  // sqrt(pi)/2
-    }
     F[0] = (R)(scale * f0);
     if constexpr (M == 1) {
         double f1;

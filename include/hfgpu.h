@@ -18,7 +18,12 @@
  *     functions ordered as hfscf orders them (shell order, Cartesian lx-major order).
  *   - Status 0 = success, negative = error; hfg_last_error() gives the message. The library
  *     never exits the process and never falls back to a CPU path.
- *   - A context owns one HIP stream and all device memory; it is not thread-safe.
+ *   - A context owns its main stream, four side streams and all its device memory; it is not
+ *     thread-safe. The class kernels that spill (private segments of several KB per lane) run
+ *     on one more stream that every context on a device shares, one launch at a time, so that a
+ *     single queue holds their scratch; contexts used from several threads on one device
+ *     therefore serialise those launches, and a HIP graph (hfg_options.graph) never captures
+ *     them (they are launched directly at each replay).
  */
 #ifndef HFGPU_H
 #define HFGPU_H
