@@ -97,8 +97,8 @@ HFG_INLINE void eri_cart(const PairDev& bp, const PairDev& kp, const double4* __
     using H = Hrr<LA, LB, LC, LD, R>;
     R acc[H::NE][H::NF];
     sfor<H::NE>([&](auto e) { sfor<H::NF>([&](auto f) { acc[e][f] = (R)0; }); });
-    if constexpr (std::is_same_v<R, float> && LT <= 2) {
-        // mixed precision, L <= 2: the whole primitive quartet in FP32 (the centre differences
+    if constexpr (std::is_same_v<R, float> && LT <= 3) {
+        // mixed precision, L <= 3: the whole primitive quartet in FP32 (the centre differences
         // P - Q in FP64 first), Boys values from boys32
         const int kn = kp.np, ks = kp.pstride;
         const float cut = (float)prim_cut;

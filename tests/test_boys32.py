@@ -1,5 +1,5 @@
 """The FP32 Boys values of the mixed-precision builds (hfscf_amd/csrc/gpu/boys32.hpp), evaluated
-on the CPU against the incomplete gamma function (scipy): F_m(T) = Gamma(m + 1/2) P(m + 1/2, T) /
+on the CPU against the incomplete gamma function (scipy), m = 0..3: F_m(T) = Gamma(m + 1/2) P(m + 1/2, T) /
 (2 T^(m + 1/2)), with the T -> 0 limit 1/(2m + 1). Bar: 2e-6 relative (FP32 arithmetic, the
 upward recursion at T >= 1 loses at most a few units of the last place per step)."""
 import ctypes as C
@@ -31,10 +31,10 @@ def _boys(m, T):
     return out
 
 
-@pytest.mark.parametrize("m", [0, 1, 2])
+@pytest.mark.parametrize("m", [0, 1, 2, 3])
 def test_boys32_against_incomplete_gamma(lib, m):
-    T = np.concatenate([[0.0, 1e-8, 1e-4], np.linspace(0.001, 1.2, 400), [0.999999, 1.0, 1.000001],
-                        np.geomspace(1.2, 120.0, 400)]).astype(np.float32)
+    T = np.concatenate([[0.0, 1e-8, 1e-4], np.linspace(0.001, 2.4, 600), [0.999999, 1.0, 1.000001, 1.999999, 2.0,
+                        2.000001, 15.99999, 16.0, 16.00001], np.geomspace(2.4, 120.0, 400)]).astype(np.float32)
     F = np.zeros((len(T), m + 1), dtype=np.float32)
     lib.boys32_eval(C.c_int(m), C.c_int(len(T)), T.ctypes.data_as(C.POINTER(C.c_float)),
                     F.ctypes.data_as(C.POINTER(C.c_float)))
