@@ -360,7 +360,7 @@ def main():
         "build_roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP64, "unit": "TFLOP/s",
                            "frac": achieved / PEAK_FP64, "model_gflop_per_build": st["model_flops"] / 1e9,
                            "kernel_ms_per_build": 1e3 * kern_per,
-                           "note": "all class launches of one build (four streams) over the event span"},
+                           "note": "all class launches of one build (concurrent streams) over the event span"},
         "classes": [{"cls": c["cls"], "quartets": c["quartets"], "model_gflop": c["model_flops"] / 1e9,
                      "kernel_ms": c["kernel_ms"],
                      "tflops": c["model_flops"] / c["kernel_ms"] / 1e9 if c["kernel_ms"] > 0 else None}

@@ -56,6 +56,7 @@ void assign_streams(hfg_ctx* c, bool grad, const std::vector<double>& w) {
     std::iota(ord.begin(), ord.end(), 0);
     std::stable_sort(ord.begin(), ord.end(), [&](size_t x, size_t y) { return w[x] > w[y]; });
     double load[kSideStreams] = {0};
+    const int nside = std::max(1, std::min(c->nside, kSideStreams));
     for (size_t k : ord) {
         const ClassWork& cw = c->classes[k];
         if (class_spills(cw.la, cw.lb, cw.lc, cw.ld, grad) && !(grad ? cw.gcoop : cw.coop)) {
@@ -63,7 +64,7 @@ void assign_streams(hfg_ctx* c, bool grad, const std::vector<double>& w) {
             continue;
         }
         int best = 0;
-        for (int s = 1; s < kSideStreams; ++s)
+        for (int s = 1; s < nside; ++s)
             if (load[s] < load[best]) best = s;
         map[k] = best;
         load[best] += w[k];
@@ -1088,7 +1089,7 @@ int hfg_create(hfg_ctx** out, int device, const hfg_options* opt) {
         c->opt = *opt;
     else
         hfg_default_options(&c->opt);
-    if ((c->opt.precision != HFG_FP64 && c->opt.precision != HFG_MIXED) ||
+    if ((c->opt.precision != HFG_FP64 && c->opt.precision != HFG_MIXED) || c->opt.side_streams < 0 ||
         (c->opt.coop_min_l != 0 && (c->opt.coop_min_l < kCoopTuneMinL || c->opt.coop_min_l > kCoopRequiredL))) {
         delete c;
         return HFG_ERR_ARG;
@@ -1107,7 +1108,8 @@ int hfg_create(hfg_ctx** out, int device, const hfg_options* opt) {
         hfg_destroy(c);
         return HFG_ERR_HIP;
     }
-    for (int k = 0; k < kSideStreams; ++k)
+    c->nside = c->opt.side_streams > 0 ? std::min(c->opt.side_streams, kSideStreams) : kDefaultSideStreams;
+    for (int k = 0; k < c->nside; ++k)
         if (hipStreamCreate(&c->side[k]) != hipSuccess ||
             hipEventCreateWithFlags(&c->ev_join[k], hipEventDisableTiming) != hipSuccess) {
             hfg_destroy(c);

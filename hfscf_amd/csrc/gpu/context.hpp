@@ -40,7 +40,12 @@ struct DevBuf {
     }
 };
 
-constexpr int kSideStreams = 4;  // class launches run concurrently on these
+constexpr int kSideStreams = 8;  // most side streams a context may use (hfg_options.side_streams)
+// Default side streams: HIP gives a process GPU_MAX_HW_QUEUES (4) hardware queues and the
+// streams share them; measured FP64 / mixed builds with 1, 2, 3, 4, 6 side streams: (H2O)16
+// 43.8/25.8, 34.3/18.0, 33.6/18.9, 34.6/18.9, 33.4/17.6 ms; (H2O)8 15.8/11.0, 10.1/6.3, 9.2/5.6,
+// 10.8/6.4, 8.9/5.5 ms.
+constexpr int kDefaultSideStreams = 6;
 // Classes from L = la+lb+lc+ld = kCoopRequiredL have only the cooperative kernels; from
 // kCoopTuneMinL up both forms exist and hfg_options.coop_min_l / hfg_autotune choose.
 constexpr int kCoopRequiredL = 7;
@@ -118,6 +123,7 @@ struct hfg_ctx {
     double* scratch = nullptr;  // the device's fixed scratch of hfg_build_jk_noalloc
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int nside = hfg::kDefaultSideStreams;  // side streams in use, class launches run concurrently on these
     hipStream_t side[hfg::kSideStreams] = {};
     hipEvent_t ev_fork = nullptr, ev_join[hfg::kSideStreams] = {};
     // the device's shared stream for kernels that may spill to scratch (see shared_scratch_stream)
@@ -238,7 +244,7 @@ inline hipError_t for_each_class_concurrent(hfg_ctx* c, hipStream_t main, F&& fn
     hipError_t e = hipEventRecord(c->ev_fork, main);
     if (e != hipSuccess) return e;
     if (side)
-        for (int k = 0; k < kSideStreams; ++k)
+        for (int k = 0; k < c->nside; ++k)
             if ((e = hipStreamWaitEvent(c->side[k], c->ev_fork, 0)) != hipSuccess) return e;
     if (shared && (e = hipStreamWaitEvent(c->heavy, c->ev_fork, 0)) != hipSuccess) return e;
     // the shared-stream classes first: they are few workgroups of long latency, and launched
@@ -253,7 +259,7 @@ inline hipError_t for_each_class_concurrent(hfg_ctx* c, hipStream_t main, F&& fn
                     return e;
             }
     if (side)
-        for (int k = 0; k < kSideStreams; ++k) {
+        for (int k = 0; k < c->nside; ++k) {
             if ((e = hipEventRecord(c->ev_join[k], c->side[k])) != hipSuccess) return e;
             if ((e = hipStreamWaitEvent(main, c->ev_join[k], 0)) != hipSuccess) return e;
         }

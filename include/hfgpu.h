@@ -18,12 +18,12 @@
  *     functions ordered as hfscf orders them (shell order, Cartesian lx-major order).
  *   - Status 0 = success, negative = error; hfg_last_error() gives the message. The library
  *     never exits the process and never falls back to a CPU path.
- *   - A context owns its main stream, four side streams and all its device memory; it is not
- *     thread-safe. The class kernels that spill (private segments of several KB per lane) run
- *     on one more stream that every context on a device shares, one launch at a time, so that a
- *     single queue holds their scratch; contexts used from several threads on one device
- *     therefore serialise those launches, and a HIP graph (hfg_options.graph) never captures
- *     them (they are launched directly at each replay).
+ *   - A context owns its main stream, its side streams (six by default) and all its device
+ *     memory; it is not thread-safe. The class kernels that spill (private segments of several
+ *     KB per lane) run on one more stream that every context on a device shares, one launch at
+ *     a time, so that a single queue holds their scratch; contexts used from several threads on
+ *     one device therefore serialise those launches, and a HIP graph (hfg_options.graph) never
+ *     captures them (they are launched directly at each replay).
  */
 #ifndef HFGPU_H
 #define HFGPU_H
@@ -44,7 +44,8 @@ enum hfg_status {
 
 enum hfg_precision {
     HFG_FP64 = 0,  /* every ERI and digestion in FP64 */
-    HFG_MIXED = 1  /* FP32 recursions/digestion, FP64 Boys, Schwarz, tile sums and Fock sum */
+    HFG_MIXED = 1  /* FP32 recursions/digestion (and the whole primitive loop up to L = 3), FP64
+                      Boys values above L = 3, Schwarz, tile sums and Fock sum */
 };
 
 typedef struct hfg_options {
@@ -64,7 +65,8 @@ typedef struct hfg_options {
                              kernels (one workgroup per quartet, recursion tables in LDS), the others
                              on the per-lane register kernels; 4..7, 0 = 7 (the default). From L = 4 up,
                              hfg_autotune keeps whichever form it measures faster for each class. */
-    int reserved[3];
+    int side_streams;     /* streams the class launches of one build are spread over (1..8, 0 = 6) */
+    int reserved[2];
 } hfg_options;
 
 /* Fill opt with defaults (FP64, integral_tol 1e-14, screening on, no density screening). */
