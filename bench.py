@@ -289,6 +289,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch shapes and FLOP-model stream balance")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--gradient-only", action="store_true",
+                    help="time only the FP64 and mixed two-electron gradients (for profiling); prints no bench line")
     ap.add_argument("--no-scf", action="store_true",
                     help="time the builds on a seeded random density instead of the converged one "
                          "(FP64 build cost does not depend on the density values; for profiler runs)")
@@ -328,6 +330,9 @@ def main():
         D = scf["D"]
         log(f"SCF {args.workload}: E = {scf['energy']:.10f} Eh (final build {scf['energy_final']:.10f}), "
             f"{scf['iterations']} iterations, {scf['seconds']:.2f} s")
+    if args.gradient_only:
+        log(json.dumps(gradient_figures(mol, D, args.steps, args.warmup, not args.no_autotune)))
+        return
     ctx = H.GpuContext(mol)
     if not args.no_autotune:
         ctx.autotune(1)  # launch shapes and stream balance from measured class times (untimed set-up)
