@@ -51,8 +51,8 @@ constexpr int kDefaultSideStreams = 6;
 constexpr int kCoopRequiredL = 7;
 constexpr int kCoopTuneMinL = 4;
 // Autotuning tries the cooperative form from this L only: below it the per-lane register kernels
-// won every measurement on (H2O)8 and (H2O)16 / cc-pVDZ (by up to 5x at L = 4), and the trials of
-// large classes cost seconds in a batch.
+// won every measurement on (H2O)8 and (H2O)16 / cc-pVDZ (by up to 5x at L = 4; the (H2O)16
+// gradient's L = 5 classes too), and the trials of large classes cost seconds in a batch.
 constexpr int kCoopTrialMinL = 6;
 
 inline int nfun(int L, bool pure) { return pure ? 2 * L + 1 : (L + 1) * (L + 2) / 2; }

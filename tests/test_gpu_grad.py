@@ -122,6 +122,10 @@ def test_h2o16_gradient_against_oracle(h2o16):
     assert abs(de) < 1e-9
     assert dg < 1e-8
     assert np.abs(g.sum(0)).max() < 1e-8
+    # rotational invariance: no torque (the oracle's own gradient has 4.9e-8 at its convergence)
+    torque = np.abs(np.cross(mol.geometry()[1], g).sum(0)).max()
+    print(f"(H2O)16 torque {torque:.1e}", flush=True)
+    assert torque < 2e-7
     fe = np.load(golden("energy_h2o16.npz"))
     assert abs(r["energy_final"] - float(fe["energy_final"])) < 1e-9
 
@@ -140,6 +144,35 @@ def test_h2o16_mixed_gradient(h2o16):
     assert abs(de) <= 1e-6
     assert np.abs(gk - g).max() <= 1e-5
     assert np.abs(gm - g).max() <= 1e-5
+
+
+def _rotation(seed):
+    q = np.random.default_rng(seed).standard_normal(4)
+    a, b, c, d = q / np.linalg.norm(q)
+    return np.array([[a * a + b * b - c * c - d * d, 2 * (b * c - a * d), 2 * (b * d + a * c)],
+                     [2 * (b * c + a * d), a * a - b * b + c * c - d * d, 2 * (c * d - a * b)],
+                     [2 * (b * d - a * c), 2 * (c * d + a * b), a * a - b * b - c * c + d * d]])
+
+
+@pytest.mark.parametrize("basis", ["cc-pvdz", "6-31g*"])
+def test_energy_and_gradient_rotation_translation_invariance(basis):
+    # the water dimer rotated by a seeded rotation and shifted: the same energy, the gradient
+    # rotated with it (unscreened builds: the screen's element skips follow Cartesian axes)
+    from oracle import basis_ref
+    Z, X = basis_ref.read_xyz(golden("h2o_8.xyz"))
+    Z, X = Z[:6], X[:6]
+    R, t = _rotation(11), np.array([0.7, -1.3, 2.1])
+    res = []
+    for xyz in (X, X @ R.T + t):
+        mol = H.Molecule.from_arrays(Z, xyz, basis)
+        r = H.rhf(mol, diis_range=8, energy_tol=1e-12, rms_tol=1e-11, schwarz=0)
+        assert r["converged"]
+        res.append((r["energy_final"], H.rhf_gradient(mol, r["D"], r["F"])))
+    (e0, g0), (e1, g1) = res
+    dg = np.abs(g1 - g0 @ R.T).max()
+    print(f"{basis}: dE {e1 - e0:.2e}, |g(R X + t) - R g(X)| {dg:.2e}", flush=True)
+    assert abs(e1 - e0) < 1e-10
+    assert dg < 1e-8
 
 
 def _h2o16_coords():

@@ -177,3 +177,26 @@ def test_cfg1_fixture_reproduces():
     o = oracle.Oracle(mol.shells(), mol.geometry()[1], mol.pure, 1e-14, screen=False)
     r = H.rhf(mol, jk=o.jk, diis_range=6, energy_tol=1e-10, rms_tol=1e-10)
     assert abs(r["energy"] - float(f["energy"])) < 1e-11
+
+
+def test_eri_pair_matrix_positive_semidefinite():
+    # (ij|kl) as a matrix over function pairs is a Gram matrix of the pair densities under the
+    # Coulomb metric, so it has no negative eigenvalue; a wrong sign or index in the recursion
+    # breaks this (cc-pVDZ water, 24 functions, every quartet from the packed ERI vector)
+    sh, X, pure, o = _water("cc-pvdz")
+    o.build_eri()
+    v = o.eri_vector()
+    n = o.nbf
+    M = np.zeros((n * n, n * n))
+
+    def idx2(a, b):
+        return a * (a + 1) // 2 + b if a >= b else b * (b + 1) // 2 + a
+
+    for i in range(n):
+        for j in range(n):
+            ij = idx2(i, j)
+            for k in range(n):
+                for l in range(n):
+                    M[i * n + j, k * n + l] = v[idx2(ij, idx2(k, l))]
+    w = np.linalg.eigvalsh(M)
+    assert w.min() > -1e-12 * w.max(), (w.min(), w.max())
