@@ -29,8 +29,8 @@ FP64, MIXED = 0, 1
 class HfgOptions(C.Structure):
     _fields_ = [("precision", C.c_int), ("integral_tol", C.c_double), ("schwarz", C.c_int),
                 ("density_tol", C.c_double), ("graph", C.c_int), ("work_split", C.c_int),
-                ("coop_min_l", C.c_int), ("side_streams", C.c_int),
-                ("reserved", C.c_int * 2)]
+                ("coop_min_l", C.c_int), ("side_streams", C.c_int), ("group_shells", C.c_int),
+                ("reserved", C.c_int * 1)]
 
 
 class HfgStats(C.Structure):
@@ -360,7 +360,7 @@ class GpuContext:
     list (hfgpu C ABI; batched matrices are concatenated per molecule)."""
 
     def __init__(self, mol, precision=FP64, integral_tol=1e-14, schwarz=True, device=0, density_tol=0.0,
-                 graph=False, work_split=0, coop_min_l=0, side_streams=0):
+                 graph=False, work_split=0, coop_min_l=0, side_streams=0, group_shells=0):
         lib = hfgpu()
         opt = HfgOptions()
         lib.hfg_default_options(C.byref(opt))
@@ -370,6 +370,7 @@ class GpuContext:
         opt.work_split = int(work_split)
         opt.coop_min_l = int(coop_min_l)
         opt.side_streams = int(side_streams)
+        opt.group_shells = int(group_shells)
         h = C.c_void_p()
         rc = lib.hfg_create(C.byref(h), device, C.byref(opt))
         if rc:

@@ -190,6 +190,143 @@ PairDev make_pair(hfg_ctx* c, int a, int b, std::vector<double4>& pp0, std::vect
     return p;
 }
 
+// Shells a and b on one atom with the same angular momentum and identical exponents.
+bool same_exponents(const hfg_ctx* c, int a, int b) {
+    if (c->atom[a] != c->atom[b] || c->L[a] != c->L[b] || c->nprim[a] != c->nprim[b]) return false;
+    for (int k = 0; k < c->nprim[a]; ++k)
+        if (c->alpha[c->prim_off[a] + k] != c->alpha[c->prim_off[b] + k]) return false;
+    return true;
+}
+
+// Shell groups of the s blocks: consecutive shells of a block on one atom with identical
+// exponents, at most kGcMaxShells per group (a block's shells keep the basis order, so the
+// contracted s shells of one atom are neighbours). For every s-s block pair, the list of group
+// pairs (GA, GB) (GB <= GA when both blocks are one) sorted by descending key: members are the
+// shell pairs (a in GA, b in GB) in canonical order (b <= a within one group), each the index
+// of its surviving per-shell pair (-1 when screened out), and the key is the largest member key,
+// so the group lists are screened like the per-shell lists and each member quartet again on its
+// own keys. The primitive pairs of a group pair are those of its exponents with one K per
+// member; the ones where some member's K passes the significance test of the per-shell primitive
+// pairs (|K| mq >= sig_cut) come first, in basis order, and np counts them. Returns false (no
+// group lists) when no group has two shells or group_shells is off.
+bool build_groups(hfg_ctx* c, double mq, double sig_cut, std::vector<double4>& gp0, std::vector<double4>& gK) {
+    static const double kSqrt2Pi25 = std::sqrt(2.0 * std::pow(M_PI, 2.5));
+    const int nbp = (int)c->bp_X.size();
+    c->gpairs.clear();
+    c->gkey.clear();
+    c->goff.assign(nbp, 0);
+    c->gcnt.assign(nbp, 0);
+    c->gc_nfull.assign(nbp, 0);
+    gp0.clear();
+    gK.clear();
+    if (c->opt.group_shells < 0) return false;
+    std::vector<std::vector<std::vector<int>>> groups(c->nblk);  // slots of each group of each s block
+    bool twins = false;
+    for (int X = 0; X < c->nblk; ++X) {
+        if (c->blk_L[X] != 0) continue;
+        const std::vector<int>& sh = c->blk_shells[X];
+        for (size_t s = 0; s < sh.size();) {
+            std::vector<int> g{(int)s};
+            while ((int)g.size() < kGcMaxShells && s + g.size() < sh.size() &&
+                   same_exponents(c, sh[s], sh[s + g.size()]))
+                g.push_back((int)(s + g.size()));
+            twins = twins || g.size() > 1;
+            s += g.size();
+            groups[X].push_back(std::move(g));
+        }
+    }
+    if (!twins) return false;
+    struct Entry {
+        double key = 0.0;
+        GcPairDev gp{};
+        std::vector<double4> p0, K;
+    };
+    std::vector<int> pidx(kBlockFuncs * kBlockFuncs);
+    std::vector<double4> rest0, restK;  // a group pair's insignificant primitive pairs
+    for (int g = 0; g < nbp; ++g) {
+        c->goff[g] = (int)c->gpairs.size();
+        const int X = c->bp_X[g], Y = c->bp_Y[g];
+        if (c->blk_L[X] != 0 || c->blk_L[Y] != 0) continue;
+        std::fill(pidx.begin(), pidx.end(), -1);
+        for (int k = c->bp_off[g]; k < c->bp_off[g] + c->bp_cnt[g]; ++k)
+            pidx[c->pairs[k].slot_a * kBlockFuncs + c->pairs[k].slot_b] = k;
+        std::vector<Entry> ents;
+        for (size_t ia = 0; ia < groups[X].size(); ++ia)
+            for (size_t ib = 0; ib < groups[Y].size(); ++ib) {
+                if (X == Y && ib > ia) continue;
+                Entry e;
+                int sha[kGcMaxMembers], shb[kGcMaxMembers];
+                bool live = false;
+                for (int sa : groups[X][ia])
+                    for (int sb : groups[Y][ib]) {
+                        if (X == Y && ia == ib && sb > sa) continue;
+                        const int m = e.gp.nmem++;
+                        e.gp.mem[m] = pidx[sa * kBlockFuncs + sb];
+                        sha[m] = c->blk_shells[X][sa];
+                        shb[m] = c->blk_shells[Y][sb];
+                        if (e.gp.mem[m] >= 0) {
+                            live = true;
+                            e.key = std::max(e.key, c->psmean[e.gp.mem[m]]);
+                        }
+                    }
+                if (!live) continue;
+                for (int m = e.gp.nmem; m < kGcMaxMembers; ++m) e.gp.mem[m] = -1;
+                const int a0 = sha[0], b0 = shb[0];
+                const double* A = &c->xyz[3 * c->atom[a0]];
+                const double* B = &c->xyz[3 * c->atom[b0]];
+                double r2 = 0;
+                for (int d = 0; d < 3; ++d) {
+                    e.gp.AB[d] = A[d] - B[d];
+                    r2 += e.gp.AB[d] * e.gp.AB[d];
+                }
+                for (int i = 0; i < c->nprim[a0]; ++i)
+                    for (int j = 0; j < c->nprim[b0]; ++j) {
+                        const double ai = c->alpha[c->prim_off[a0] + i], bj = c->alpha[c->prim_off[b0] + j];
+                        const double p = ai + bj;
+                        double K[kGcMaxMembers] = {0.0, 0.0, 0.0, 0.0};
+                        bool sig = false;
+                        for (int m = 0; m < e.gp.nmem; ++m) {
+                            if (e.gp.mem[m] < 0) continue;
+                            // the per-shell primitive pair's K, evaluated in the same order
+                            K[m] = std::exp(-ai * bj / p * r2) * c->coef[c->prim_off[sha[m]] + i] *
+                                   c->coef[c->prim_off[shb[m]] + j] * kSqrt2Pi25 / p;
+                            sig = sig || std::fabs(K[m]) * mq >= sig_cut;
+                        }
+                        double P[3];
+                        for (int d = 0; d < 3; ++d) P[d] = (ai * A[d] + bj * B[d]) / p;
+                        (sig ? e.p0 : rest0).push_back(make_double4(p, P[0], P[1], P[2]));
+                        (sig ? e.K : restK).push_back(make_double4(K[0], K[1], K[2], K[3]));
+                    }
+                e.gp.np = (int)e.p0.size();
+                e.p0.insert(e.p0.end(), rest0.begin(), rest0.end());
+                e.K.insert(e.K.end(), restK.begin(), restK.end());
+                rest0.clear();
+                restK.clear();
+                ents.push_back(std::move(e));
+            }
+        std::stable_sort(ents.begin(), ents.end(), [](const Entry& x, const Entry& y) { return x.key > y.key; });
+        // primitive-major: every group pair of the list has nprim_X x nprim_Y primitive pairs
+        const size_t base = gp0.size(), cnt = ents.size();
+        const size_t nfull = ents.empty() ? 0 : ents[0].p0.size();
+        gp0.resize(base + nfull * cnt);
+        gK.resize(gp0.size());
+        for (size_t j = 0; j < cnt; ++j) {
+            Entry& e = ents[j];
+            for (size_t k = 0; k < nfull; ++k) {
+                gp0[base + k * cnt + j] = e.p0[k];
+                gK[base + k * cnt + j] = e.K[k];
+            }
+            e.gp.off = (int)(base + j);
+            e.gp.pstride = (int)cnt;
+            c->gpairs.push_back(e.gp);
+            c->gkey.push_back(e.key);
+        }
+        c->gcnt[g] = (int)c->gpairs.size() - c->goff[g];
+        c->gc_nfull[g] = (int)nfull;
+    }
+    return true;
+}
+
 void destroy_graphs(hfg_ctx* c) {
     for (auto& g : c->graphs)
         if (g.exec) (void)hipGraphExecDestroy(g.exec);
@@ -236,10 +373,11 @@ int setup_geometry(hfg_ctx* c) {
     // (which keep their basis order: the FP32 kernels' primitive sums are sensitive to the order,
     // and a largest-first order made them 30 times less accurate) and the loop count np stops
     // before them; the storage keeps all of them.
+    double mq = 0.0;
+    for (size_t k = 0; k < pp0.size(); ++k) mq = std::max(mq, std::fabs(pp1[k].w) / std::sqrt(pp0[k].x));
+    const double sig_cut = std::min(1e-16, c->opt.integral_tol / 1e4);
     {
-        double mq = 0.0;
-        for (size_t k = 0; k < pp0.size(); ++k) mq = std::max(mq, std::fabs(pp1[k].w) / std::sqrt(pp0[k].x));
-        const double cut = std::min(1e-16, c->opt.integral_tol / 1e4);
+        const double cut = sig_cut;
         std::vector<int> perm;
         std::vector<double4> t0, t1, t2;
         for (auto& v : cand)
@@ -434,6 +572,33 @@ int setup_geometry(hfg_ctx* c) {
         HFG_CHECK(c, c->d_pp1.upload(q1, c->stream));
         HFG_CHECK(c, c->d_pp2.upload(q2, c->stream));
     }
+    // shell groups of the s blocks and their group-pair lists
+    std::vector<double4> gp0, gK;
+    const bool gc_on = build_groups(c, mq, sig_cut, gp0, gK);
+    std::vector<double> gcum_np, gcum_npnm, gcum_nm;  // per block pair prefixes over its group list
+    if (gc_on) {
+        HFG_CHECK(c, c->d_gpairs.upload(c->gpairs, c->stream));
+        HFG_CHECK(c, c->d_gkey.upload(c->gkey, c->stream));
+        HFG_CHECK(c, c->d_goff.upload(c->goff, c->stream));
+        HFG_CHECK(c, c->d_gcnt.upload(c->gcnt, c->stream));
+        HFG_CHECK(c, c->d_gc_p0.upload(gp0, c->stream));
+        HFG_CHECK(c, c->d_gc_K.upload(gK, c->stream));
+        c->gc_args = GcArgs{c->d_gpairs.p, c->d_gkey.p, c->d_goff.p, c->d_gcnt.p, c->d_gc_p0.p, c->d_gc_K.p};
+        gcum_np.assign(c->gpairs.size() + nbp, 0.0);
+        gcum_npnm.assign(gcum_np.size(), 0.0);
+        gcum_nm.assign(gcum_np.size(), 0.0);
+        for (int P = 0; P < nbp; ++P) {
+            const int k0 = c->goff[P] + P;  // list P's prefixes start at goff[P] + P
+            for (int i = 0; i < c->gcnt[P]; ++i) {
+                const GcPairDev& gp = c->gpairs[c->goff[P] + i];
+                gcum_np[k0 + i + 1] = gcum_np[k0 + i] + gp.np;
+                gcum_npnm[k0 + i + 1] = gcum_npnm[k0 + i] + (double)gp.np * gp.nmem;
+                gcum_nm[k0 + i + 1] = gcum_nm[k0 + i] + gp.nmem;
+            }
+        }
+    } else {
+        c->gc_args = GcArgs{};
+    }
     // work items per class
     std::vector<double> cum_np(c->pairs.size() + nbp, 0.0);  // per block pair prefix of np
     std::vector<int> cum_off(nbp, 0);
@@ -486,6 +651,27 @@ int setup_geometry(hfg_ctx* c) {
                     class_cost[ci] += cost;
                 }
             }
+    // (ss|ss) on the group lists: ket count and cost of group row i of list P against list Q
+    // (model: per group primitive quartet the prefactor and Boys value of one primitive quartet plus
+    // a multiply, a compare and an add per member pair; per member shell quartet the digestion)
+    auto gc_row = [&](int P, int Q, int i, const FlopParts& fp) {
+        int cnt = ket_count_host(&c->gkey[c->goff[Q]], c->gcnt[Q], c->gkey[c->goff[P] + i], c->tol);
+        if (P == Q) cnt = std::min(cnt, i + 1);
+        const GcPairDev& gb = c->gpairs[c->goff[P] + i];
+        const int k0 = c->goff[Q] + Q;
+        const double cost = gb.np * (fp.per_prim * gcum_np[k0 + cnt] + 3.0 * gb.nmem * gcum_npnm[k0 + cnt]) +
+                            fp.per_quartet * gb.nmem * gcum_nm[k0 + cnt];
+        return std::make_pair(cnt, cost);
+    };
+    double gc_cost = 0;  // the (ss|ss) class's cost on the group lists
+    if (gc_on) {
+        const FlopParts fp = model_flops(0, 0, 0, 0, c->pure, false);
+        for (const auto& M : c->mols)
+            for (int P = M.bp0; P < M.bp0 + M.nbp; ++P)
+                for (int Q = M.bp0; Q <= P; ++Q)
+                    if (c->gcnt[P] && c->gcnt[Q])
+                        for (int i = 0; i < c->gcnt[P]; ++i) gc_cost += gc_row(P, Q, i, fp).second;
+    }
     const double split = c->opt.work_split > 0 ? (double)c->opt.work_split : kDefaultWorkSplit;
     const double target = std::max(total_cost / (256.0 * 6.0 * split), 1.0);
     c->slab_doubles = 0;
@@ -493,6 +679,37 @@ int setup_geometry(hfg_ctx* c) {
     c->total_quartets = 0;
     c->total_flops = 0;
     c->total_grad_flops = 0;
+    // Work items of the rows [0, nrows) of a bra list against a ket list: consecutive rows until
+    // their cost reaches tgt (or kMaxRows rows), an expensive row as several items over chunks of
+    // its ket columns. row(i) gives {kets of row i, cost of row i}; push(r0, r1, c0, c1) adds one.
+    auto split_rows = [](int nrows, double tgt, auto&& row, auto&& push) {
+        double acc = 0;
+        long long q_acc = 0;
+        int r0 = -1;
+        auto flush = [&](int r1) {
+            if (r0 >= 0 && q_acc > 0) push(r0, r1, 0, 0x7fffffff);
+            r0 = -1;
+            acc = 0;
+            q_acc = 0;
+        };
+        for (int i = 0; i < nrows; ++i) {
+            const std::pair<int, double> rc = row(i);
+            const int cnt = rc.first;
+            const double row_cost = rc.second;
+            if (row_cost > tgt && cnt > 64) {
+                flush(i);
+                const int chunks = std::min((int)std::ceil(row_cost / tgt), (cnt + 31) / 32);
+                const int width = (cnt + chunks - 1) / chunks;
+                for (int j0 = 0; j0 < cnt; j0 += width) push(i, i + 1, j0, std::min(cnt, j0 + width));
+                continue;
+            }
+            if (r0 < 0) r0 = i;
+            acc += row_cost;
+            q_acc += cnt;
+            if (acc >= tgt || i + 1 - r0 >= kMaxRows) flush(i + 1);
+        }
+        flush(nrows);
+    };
     for (int m = 0; m < (int)c->mols.size(); ++m) {
         const MolHost& M = c->mols[m];
         for (int P = M.bp0; P < M.bp0 + M.nbp; ++P)
@@ -508,7 +725,11 @@ int setup_geometry(hfg_ctx* c) {
                                  c->blk_nf[B] * c->blk_nf[C] + c->blk_nf[B] * c->blk_nf[D];
                 // per-class item target: a class launch has >= 256 x split items when it has the work
                 const double tgt = std::max(std::min(target, class_cost[ci] / (256.0 * split)), 1.0);
+                // (ss|ss) on the group lists: the J/K items come from those, the per-shell items
+                // below serve the gradient only
+                const bool on_groups = gc_on && cw.la + cw.lb + cw.lc + cw.ld == 0 && c->gcnt[P] && c->gcnt[Q];
                 WorkItem w{};
+                w.split = 1;
                 w.P = P;
                 w.Q = Q;
                 w.A = A;
@@ -522,50 +743,52 @@ int setup_geometry(hfg_ctx* c) {
                     v.r1 = r1;
                     v.c0 = c0;
                     v.c1 = c1;
-                    v.slab = c->slab_doubles;
-                    c->slab_doubles += tile;
+                    v.slab = on_groups ? -1 : c->slab_doubles;
+                    if (!on_groups) c->slab_doubles += tile;
                     v.gslab = c->grad_slab_doubles;
                     c->grad_slab_doubles += 3 * M.natom;
                     cw.items.push_back(v);
                 };
-                double acc = 0;
-                long long q_acc = 0;
-                int r0 = -1;
-                auto flush = [&](int r1) {
-                    if (r0 >= 0 && q_acc > 0) push(r0, r1, 0, 0x7fffffff);
-                    r0 = -1;
-                    acc = 0;
-                    q_acc = 0;
-                };
-                for (int i = 0; i < c->bp_cnt[P]; ++i) {
-                    int cnt =
-                        ket_count_host(&c->psmean[c->bp_off[Q]], c->bp_cnt[Q], c->psmean[c->bp_off[P] + i], c->tol);
+                split_rows(c->bp_cnt[P], tgt, [&](int i) {
+                    int cnt = ket_count_host(&c->psmean[c->bp_off[Q]], c->bp_cnt[Q], c->psmean[c->bp_off[P] + i], c->tol);
                     if (P == Q) cnt = std::min(cnt, i + 1);
                     const double np_i = c->pairs[c->bp_off[P] + i].np;
                     const double kq = np_i * cum_np[cum_off[Q] + cnt];
                     cw.quartets += cnt;
                     cw.flops += kq * fp.per_prim + cnt * fp.per_quartet;
                     cw.grad_flops += kq * gp.per_prim + cnt * gp.per_quartet;
-                    const double row_cost = kq * fp.per_prim + cnt * fp.per_quartet;
-                    if (row_cost > tgt && cnt > 64) {
-                        // an expensive row becomes several items over ket column chunks
-                        flush(i);
-                        const int chunks = std::min((int)std::ceil(row_cost / tgt), (cnt + 31) / 32);
-                        const int width = (cnt + chunks - 1) / chunks;
-                        for (int j0 = 0; j0 < cnt; j0 += width) push(i, i + 1, j0, std::min(cnt, j0 + width));
-                        continue;
-                    }
-                    if (r0 < 0) r0 = i;
-                    acc += row_cost;
-                    q_acc += cnt;
-                    if (acc >= tgt || i + 1 - r0 >= kMaxRows) flush(i + 1);
-                }
-                flush(c->bp_cnt[P]);
+                    return std::make_pair(cnt, kq * fp.per_prim + cnt * fp.per_quartet);
+                }, push);
+                if (!on_groups) continue;
+                cw.gc = true;
+                // lanes per group quartet: enough that one lane evaluates at most about
+                // kGcLanePrims primitive quartets (the lists' full primitive counts)
+                const double pq = (double)c->gc_nfull[P] * c->gc_nfull[Q];
+                const int lanes = std::max(1, std::min(kGcMaxSplit, (int)std::ceil(pq / kGcLanePrims)));
+                auto push_gc = [&](int r0, int r1, int c0, int c1) {
+                    WorkItem v = w;
+                    v.split = lanes;
+                    v.r0 = r0;
+                    v.r1 = r1;
+                    v.c0 = c0;
+                    v.c1 = c1;
+                    v.slab = c->slab_doubles;
+                    c->slab_doubles += tile;
+                    v.gslab = -1;
+                    cw.gc_items.push_back(v);
+                };
+                // as many items as the per-shell lists would give the class (its items are cheaper)
+                const double tgt_gc = std::max(tgt * gc_cost / std::max(class_cost[ci], 1.0), 1.0);
+                split_rows(c->gcnt[P], tgt_gc, [&](int i) {
+                    const std::pair<int, double> rc = gc_row(P, Q, i, fp);
+                    cw.gc_flops += rc.second;
+                    return rc;
+                }, push_gc);
             }
     }
     for (auto& cw : c->classes) {
         c->total_quartets += cw.quartets;
-        c->total_flops += cw.flops;
+        c->total_flops += cw.jk_flops();
         c->total_grad_flops += cw.grad_flops;
     }
     c->classes.erase(std::remove_if(c->classes.begin(), c->classes.end(), [](const ClassWork& w) { return w.items.empty(); }),
@@ -574,7 +797,8 @@ int setup_geometry(hfg_ctx* c) {
     {
         std::vector<size_t> ord(c->classes.size());
         std::iota(ord.begin(), ord.end(), 0);
-        std::sort(ord.begin(), ord.end(), [&](size_t x, size_t y) { return c->classes[x].flops > c->classes[y].flops; });
+        std::sort(ord.begin(), ord.end(),
+                  [&](size_t x, size_t y) { return c->classes[x].jk_flops() > c->classes[y].jk_flops(); });
         std::vector<ClassWork> sorted;
         for (size_t k : ord) sorted.push_back(c->classes[k]);
         c->classes.swap(sorted);
@@ -644,7 +868,7 @@ int setup_geometry(hfg_ctx* c) {
         }
         std::vector<double> wj, wg;
         for (const auto& cw : c->classes) {
-            wj.push_back(cw.flops);
+            wj.push_back(cw.jk_flops());
             wg.push_back(cw.grad_flops);
         }
         assign_streams(c, false, wj);
@@ -660,6 +884,10 @@ int setup_geometry(hfg_ctx* c) {
         for (auto& w : cw.items) {
             mitems[w.mol].push_back((int)all.size());
             all.push_back(w);
+        }
+        cw.gc_dev_off = all.size();
+        all.insert(all.end(), cw.gc_items.begin(), cw.gc_items.end());
+        for (auto& w : cw.jk_items()) {
             const int f[4] = {c->blk_nf[w.A], c->blk_nf[w.B], c->blk_nf[w.C], c->blk_nf[w.D]};
             const int blk[4] = {w.A, w.B, w.C, w.D};
             // (row block, col block) of Jab Jcd Kac Kad Kbc Kbd
@@ -679,7 +907,7 @@ int setup_geometry(hfg_ctx* c) {
                 off += (long long)f[rr[r]] * f[cc[r]];
             }
         }
-        c->total_items += (long long)cw.items.size();
+        c->total_items += (long long)cw.jk_items().size();
     }
     c->tiles.clear();
     c->gJ_off.assign(1, 0);
@@ -854,7 +1082,7 @@ int enqueue_classes(hfg_ctx* c, hipStream_t main, bool dscreen, bool timed, Clas
             shared_zeroed = true;
         }
         JkArgs b = a;
-        b.items = c->d_items.p + cw.dev_off;
+        b.items = c->d_items.p + cw.jk_dev_off();
         b.queue = buf.queue + qi;
         if (timed) {
             if (!cw.ev_start && (e = hipEventCreate(&cw.ev_start)) != hipSuccess) return e;
@@ -1042,6 +1270,7 @@ void layout_molecules(hfg_ctx* c) {
 
 hipError_t hfg::launch_jk_class(hfg_ctx* c, const ClassWork& cw, const JkArgs& a, LaunchShape sh, hipStream_t st) {
     const int mixed = c->opt.precision == HFG_MIXED ? 1 : 0;
+    if (cw.gc) return launch_jk_gc_ssss(a, c->gc_args, mixed, (int)cw.gc_items.size(), sh, st);
     if (cw.coop) return launch_jk_coop(a, cw.coop_args, cw.coop_lds[mixed], mixed, (int)cw.items.size(), sh, st);
     return launch_jk(cw.la, cw.lb, cw.lc, cw.ld, c->pure, mixed, a, (int)cw.items.size(), sh, st);
 }
@@ -1090,6 +1319,7 @@ int hfg_create(hfg_ctx** out, int device, const hfg_options* opt) {
     else
         hfg_default_options(&c->opt);
     if ((c->opt.precision != HFG_FP64 && c->opt.precision != HFG_MIXED) || c->opt.side_streams < 0 ||
+        c->opt.group_shells < -1 || c->opt.group_shells > 1 ||
         (c->opt.coop_min_l != 0 && (c->opt.coop_min_l < kCoopTuneMinL || c->opt.coop_min_l > kCoopRequiredL))) {
         delete c;
         return HFG_ERR_ARG;
@@ -1385,7 +1615,7 @@ int hfg_launch_shapes(const hfg_ctx* c, int cap, int* la_lb_lc_ld, int* jk_grid,
         const ClassWork& cw = c->classes[k];
         if (la_lb_lc_ld) la_lb_lc_ld[k] = cw.la * 1000 + cw.lb * 100 + cw.lc * 10 + cw.ld;
         if (jk_grid) jk_grid[k] = cw.jk_shape.grid;
-        if (jk_block) jk_block[k] = cw.coop ? -1 : cw.jk_shape.block;
+        if (jk_block) jk_block[k] = cw.gc ? -2 : cw.coop ? -1 : cw.jk_shape.block;
         if (grad_grid) grad_grid[k] = cw.grad_shape.grid;
         if (grad_block) grad_block[k] = cw.gcoop ? -1 : cw.grad_shape.block;
     }
@@ -1401,7 +1631,7 @@ int hfg_class_stats(hfg_ctx* c, int cap, int* la_lb_lc_ld, long long* quartets, 
         ClassWork& cw = c->classes[k];
         if (la_lb_lc_ld) la_lb_lc_ld[k] = cw.la * 1000 + cw.lb * 100 + cw.lc * 10 + cw.ld;
         if (quartets) quartets[k] = cw.quartets;
-        if (model_flops) model_flops[k] = cw.flops;
+        if (model_flops) model_flops[k] = cw.jk_flops();
         if (kernel_ms) {
             float ms = -1.0f;
             if (c->class_timed && cw.ev_start && cw.ev_stop) HFG_CHECK(c, hipEventElapsedTime(&ms, cw.ev_start, cw.ev_stop));
@@ -1574,6 +1804,12 @@ void hfg_destroy(hfg_ctx* c) {
     c->d_cJ_off.release();
     c->d_cK_off.release();
     c->d_gpart.release();
+    c->d_gpairs.release();
+    c->d_gkey.release();
+    c->d_goff.release();
+    c->d_gcnt.release();
+    c->d_gc_p0.release();
+    c->d_gc_K.release();
     c->d_gJ_off.release();
     c->d_gK_off.release();
     c->d_shell_atom.release();

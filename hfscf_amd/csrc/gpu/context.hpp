@@ -10,6 +10,7 @@
 #include "../../../include/hfgpu.h"
 #include "coop_args.hpp"
 #include "gather.hpp"
+#include "gc.hpp"
 #include "kernels.hpp"
 #include "launch_bounds.hpp"
 
@@ -79,6 +80,16 @@ struct ClassWork {
     bool gcoop_ok = false;
     CoopGradArgs gcoop_args{};
     size_t gcoop_lds[2] = {0, 0};
+    // J/K of (ss|ss) on the shell-group kernel: its own work items over the group lists (the
+    // per-shell items above then serve the gradient only) and the model FLOPs of that kernel
+    bool gc = false;
+    std::vector<WorkItem> gc_items;
+    size_t gc_dev_off = 0;
+    double gc_flops = 0;
+    // the work items, their offset in d_items and the model FLOPs of the J/K launch
+    const std::vector<WorkItem>& jk_items() const { return gc ? gc_items : items; }
+    size_t jk_dev_off() const { return gc ? gc_dev_off : dev_off; }
+    double jk_flops() const { return gc ? gc_flops : flops; }
 };
 
 // One molecule of a context (a context holds one, or many in batching mode).
@@ -159,6 +170,12 @@ struct hfg_ctx {
     std::vector<double> psq;  // sqrt|(pq|pq)| per pair component
     std::vector<int> bp_off, bp_cnt;
     std::vector<double> screen;  // Cartesian (pq|pq) of molecule 0 (single-molecule contexts)
+    // shell groups of the s blocks (hfg_options.group_shells): per s-s block pair the sorted list
+    // of group pairs, their keys and primitive data
+    std::vector<hfg::GcPairDev> gpairs;
+    std::vector<double> gkey;
+    std::vector<int> goff, gcnt;
+    std::vector<int> gc_nfull;  // primitive pairs stored per group pair of each list
     double tol = 0.0;       // quartets with psmean_ab psmean_cd below this are skipped
     double elem_tol = 0.0;  // hfscf's element screen and Cartesian diagonal (0 without screening)
     // work
@@ -189,6 +206,11 @@ struct hfg_ctx {
     DevBuf<GatherChunk> d_gchunks;
     DevBuf<int> d_cJ_off, d_cK_off;
     DevBuf<double> d_gpart;
+    DevBuf<hfg::GcPairDev> d_gpairs;
+    DevBuf<double> d_gkey;
+    DevBuf<int> d_goff, d_gcnt;
+    DevBuf<double4> d_gc_p0, d_gc_K;
+    hfg::GcArgs gc_args{};  // device pointers of the group tables
     // launch graphs of the J/K build (keyed by precision and density screening)
     std::vector<hfg::JkGraph> graphs;
     bool class_timed = false;  // the last J/K build bracketed each class launch with events

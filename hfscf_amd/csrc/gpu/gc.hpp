@@ -1,0 +1,49 @@
+// Shells that share exponents: pair groups and the (ss|ss) J/K kernel that evaluates each
+// group's primitive quartets once for every member shell quartet.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+#include "layout.hpp"
+
+namespace hfg {
+
+// Most shells of one atom and angular momentum with identical exponents that form one group.
+constexpr int kGcMaxShells = 2;
+constexpr int kGcMaxMembers = kGcMaxShells * kGcMaxShells;
+// Lanes per group quartet (WorkItem::split): enough that one lane evaluates about kGcLanePrims
+// primitive quartets, at most kGcMaxSplit.
+constexpr double kGcLanePrims = 512.0;
+constexpr int kGcMaxSplit = 16;
+
+// A pair of shell groups (a group of a block, a group of a block): up to four member shell
+// pairs (indices into the per-shell pair array, -1 where the pair did not survive the screen),
+// the group's primitive pairs (shared exponents and centres; one K per member in gK) and A - B.
+// Primitive pair k of a group pair is at off + k * pstride: the group pairs of one block pair's
+// list have the same primitive count and are stored primitive-major, so the lanes of a wavefront,
+// which hold consecutive group pairs of a list, read consecutive records; the np significant
+// primitive pairs come first.
+struct GcPairDev {
+    int nmem;
+    int mem[kGcMaxMembers];
+    int off, np;
+    int pstride;
+    double AB[3];
+};
+
+// Device tables of the group lists, beside JkArgs (whose items index them for the classes run
+// on the group kernel).
+struct GcArgs {
+    const GcPairDev* gpairs;  // the group lists of every block pair
+    const double* gkey;       // sort key of each group (the largest of its members' keys)
+    const int* goff;          // first group of each block pair's list
+    const int* gcnt;          // groups in each block pair's list
+    const double4* gp0;       // per group primitive pair: {p, P}
+    const double4* gK;        // per group primitive pair: K of members 0..3
+};
+
+// The (ss|ss) class on the group lists (mixed != 0: the FP32 primitive loop of the mixed builds).
+hipError_t launch_jk_gc_ssss(const JkArgs& a, const GcArgs& g, int mixed, int nitems, LaunchShape sh,
+                             hipStream_t s);
+
+}  // namespace hfg

@@ -121,7 +121,7 @@ int run_one_class(hfg_ctx* c, size_t k, bool grad, LaunchShape sh) {
         HFG_CHECK(c, launch_grad_any(c, cw, b, sh, mixed, st));
     } else {
         JkArgs b = jk_args(c);
-        b.items = c->d_items.p + cw.dev_off;
+        b.items = c->d_items.p + cw.jk_dev_off();
         b.queue = q;
         HFG_CHECK(c, launch_jk_class(c, cw, b, sh, st));
     }

@@ -44,7 +44,8 @@ struct WorkItem {
     int c0, c1;        // ket columns [c0, c1) of each row (clipped to the row's surviving range)
     int A, B, C, D;    // blocks
     int mol;           // molecule (a context may hold many independent molecules)
-    int pad_;
+    int split;         // lanes per quartet: the shell-group kernel splits each group quartet's bra
+                       // primitive pairs over this many lanes (1 in every other kernel)
     long long slab;    // offset (doubles) of this item's six tiles
     long long gslab;   // offset (doubles) of this item's gradient partials (3 per atom of mol)
 };

@@ -66,7 +66,12 @@ typedef struct hfg_options {
                              on the per-lane register kernels; 4..7, 0 = 7 (the default). From L = 4 up,
                              hfg_autotune keeps whichever form it measures faster for each class. */
     int side_streams;     /* streams the class launches of one build are spread over (1..8, 0 = 6) */
-    int reserved[2];
+    int group_shells;     /* 0 or 1 (the default): s shells of one atom with identical exponents (the
+                             contracted s shells of cc-pVXZ, which share one set of primitives) form
+                             groups, and the J/K (ss|ss) class evaluates each primitive quartet of a
+                             group quartet once for all of its member shell quartets; -1 = off. The
+                             results are those of the per-shell kernel to rounding. */
+    int reserved[1];
 } hfg_options;
 
 /* Fill opt with defaults (FP64, integral_tol 1e-14, screening on, no density screening). */
@@ -184,7 +189,8 @@ int hfg_batch_layout(const hfg_ctx* ctx, int* nmol, int* nbf, long long* moff, i
 int hfg_autotune(hfg_ctx* ctx, int what);
 /* Current launch shape per class (la*1000+lb*100+lc*10+ld, grid, block for J/K and gradient;
  * grid 0 = one workgroup per work item, block 0 = the default, block -1 = the class runs on the
- * cooperative kernel, whose workgroup size is fixed). Returns the number of classes. */
+ * cooperative kernel, whose workgroup size is fixed, J/K block -2 = the (ss|ss) class runs on the
+ * shell-group kernel, hfg_options.group_shells). Returns the number of classes. */
 int hfg_launch_shapes(const hfg_ctx* ctx, int cap, int* la_lb_lc_ld, int* jk_grid, int* jk_block, int* grad_grid,
                       int* grad_block);
 

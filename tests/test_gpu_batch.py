@@ -80,8 +80,9 @@ def test_autotune_keeps_results():
     g0 = ctx.energy_backward(D)
     ctx.autotune(3)
     shapes = ctx.launch_shapes()
-    # tuned register kernels report their block, classes on the cooperative kernels -1
-    assert shapes and all(s["jk"][1] in (-1, 128, 256) and s["grad"][1] in (-1, 128, 256) for s in shapes)
+    # tuned register kernels report their block, classes on the cooperative kernels -1, the
+    # (ss|ss) J/K launch on the shell-group kernel -2
+    assert shapes and all(s["jk"][1] in (-2, -1, 128, 256) and s["grad"][1] in (-1, 128, 256) for s in shapes)
     assert all(s["jk"][1] == -1 for s in shapes if sum(int(x) for x in s["cls"]) >= 7)
     print("autotuned shapes:", shapes, flush=True)
     J1, K1 = ctx.build_jk(D)

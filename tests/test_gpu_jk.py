@@ -209,3 +209,49 @@ def test_jk_cooperative_kernels_from_l4(basis, dimer):
     assert np.abs(J - Jr).max() < 1e-11 and np.abs(K - Kr).max() < 1e-11
     with pytest.raises(RuntimeError):
         H.GpuContext(mol, coop_min_l=3)
+
+
+@pytest.mark.parametrize("precision", [H.FP64, H.MIXED])
+@pytest.mark.parametrize("screen", [True, False])
+def test_jk_shell_groups_match_per_shell(precision, screen):
+    # cc-pVDZ oxygen's two contracted s shells share their nine exponents, so the (ss|ss) class
+    # runs on the shell-group kernel; its J/K must be the per-shell kernel's (group_shells=-1) to
+    # rounding, and the oracle's in FP64
+    mol = H.Molecule.from_arrays(*_dimer(), "cc-pvdz")
+    D = _rand_density(mol.nbf, 9)
+    ctx = H.GpuContext(mol, precision=precision, schwarz=screen)
+    ref = H.GpuContext(mol, precision=precision, schwarz=screen, group_shells=-1)
+    assert [s["jk"][1] for s in ctx.launch_shapes() if s["cls"] == "0000"] == [-2]
+    assert all(s["jk"][1] != -2 for s in ref.launch_shapes())
+    J, K = ctx.build_jk(D)
+    J0, K0 = ref.build_jk(D)
+    dJ, dK = np.abs(J - J0).max(), np.abs(K - K0).max()
+    orc = oracle.Oracle(mol.shells(), mol.geometry()[1], mol.pure, 1e-14, screen=screen)
+    Jr, Kr = orc.jk(D)
+    eJ, eK = np.abs(J - Jr).max(), np.abs(K - Kr).max()
+    print(f"precision {precision} screen {screen}: groups vs per-shell |dJ| {dJ:.2e} |dK| {dK:.2e}; "
+          f"vs oracle |dJ| {eJ:.2e} |dK| {eK:.2e} (per-shell {np.abs(J0 - Jr).max():.2e} "
+          f"{np.abs(K0 - Kr).max():.2e})", flush=True)
+    if precision == H.FP64:
+        assert dJ < 1e-12 and dK < 1e-12
+        assert eJ < 1e-11 and eK < 1e-11
+    else:
+        # FP32 primitive quartets: no worse than the per-shell mixed kernel's error
+        assert max(eJ, eK) < 2.0 * max(np.abs(J0 - Jr).max(), np.abs(K0 - Kr).max(), 1e-9)
+    assert np.abs(J - J.T).max() == 0.0 and np.abs(K - K.T).max() == 0.0
+
+
+def test_fock_incremental_shell_groups():
+    # the density-screened incremental build on the group kernel against the per-shell kernel
+    mol = H.Molecule.from_arrays(*_dimer(), "cc-pvdz")
+    S, T, V = mol.oneint()
+    Hc = T + V
+    D1 = _rand_density(mol.nbf, 5)
+    D2 = D1 + 1e-4 * _rand_density(mol.nbf, 6)
+    out = []
+    for gs in (0, -1):
+        ctx = H.GpuContext(mol, density_tol=1e-10, group_shells=gs)
+        ctx.fock(Hc, D1)
+        out.append(ctx.fock(Hc, D2, incremental=True))
+    print(f"incremental Fock, groups vs per-shell: {np.abs(out[0] - out[1]).max():.2e}", flush=True)
+    assert np.abs(out[0] - out[1]).max() < 1e-12
