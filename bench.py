@@ -163,8 +163,8 @@ def dominant_roofline(classes, workload):
     c = max(classes, key=lambda c: c["kernel_ms"])
     cls = "".join(str(x) for x in c["cls"])
     ach = c["model_flops"] / (c["kernel_ms"] * 1e-3) / 1e12
-    traffic, entry = load_traffic(f"jk_kernel_{cls}_fp64", workload)
-    out = {"bound": "valu", "kernel": f"jk_kernel ({cls[:2]}|{cls[2:]}) fp64", "achieved": ach, "peak": PEAK_FP64,
+    traffic, entry = load_traffic(f"{c['kernel']}_{cls}_fp64", workload)
+    out = {"bound": "valu", "kernel": f"{c['kernel']} ({cls[:2]}|{cls[2:]}) fp64", "achieved": ach, "peak": PEAK_FP64,
            "unit": "TFLOP/s", "frac": ach / PEAK_FP64, "traffic": traffic,
            "model_gflop_per_launch": c["model_flops"] / 1e9, "kernel_ms": c["kernel_ms"],
            "quartets_per_launch": c["quartets"]}
@@ -339,6 +339,10 @@ def main():
     el, kern = time_builds(ctx, Hc, D, args.steps, args.warmup)
     st = ctx.stats()
     classes = ctx.class_stats()  # the last build's class launches, bracketed by HIP events
+    # kernel form of each class's J/K launch: shell-group lists, cooperative, or per-lane register
+    form = {s["cls"]: {-2: "jk_gc", -1: "jk_coop"}.get(s["jk"][1], "jk_kernel") for s in ctx.launch_shapes()}
+    for c in classes:
+        c["kernel"] = form.get("".join(str(x) for x in c["cls"]), "jk_kernel")
     ctx.close()
     builds_s = args.steps / el
     kern_per = kern / args.steps
@@ -366,8 +370,8 @@ def main():
                            "frac": achieved / PEAK_FP64, "model_gflop_per_build": st["model_flops"] / 1e9,
                            "kernel_ms_per_build": 1e3 * kern_per,
                            "note": "all class launches of one build (concurrent streams) over the event span"},
-        "classes": [{"cls": c["cls"], "quartets": c["quartets"], "model_gflop": c["model_flops"] / 1e9,
-                     "kernel_ms": c["kernel_ms"],
+        "classes": [{"cls": c["cls"], "kernel": c["kernel"], "quartets": c["quartets"],
+                     "model_gflop": c["model_flops"] / 1e9, "kernel_ms": c["kernel_ms"],
                      "tflops": c["model_flops"] / c["kernel_ms"] / 1e9 if c["kernel_ms"] > 0 else None}
                     for c in sorted(classes, key=lambda c: -c["kernel_ms"])],
     }

@@ -209,7 +209,8 @@ bool same_exponents(const hfg_ctx* c, int a, int b) {
 // member; the ones where some member's K passes the significance test of the per-shell primitive
 // pairs (|K| mq >= sig_cut) come first, in basis order, and np counts them. Returns false (no
 // group lists) when no group has two shells or group_shells is off.
-bool build_groups(hfg_ctx* c, double mq, double sig_cut, std::vector<double4>& gp0, std::vector<double4>& gK) {
+bool build_groups(hfg_ctx* c, double mq, double sig_cut, std::vector<double4>& gp0, std::vector<double4>& gp1,
+                  std::vector<double4>& gK) {
     static const double kSqrt2Pi25 = std::sqrt(2.0 * std::pow(M_PI, 2.5));
     const int nbp = (int)c->bp_X.size();
     c->gpairs.clear();
@@ -218,16 +219,16 @@ bool build_groups(hfg_ctx* c, double mq, double sig_cut, std::vector<double4>& g
     c->gcnt.assign(nbp, 0);
     c->gc_nfull.assign(nbp, 0);
     gp0.clear();
+    gp1.clear();
     gK.clear();
     if (c->opt.group_shells < 0) return false;
-    std::vector<std::vector<std::vector<int>>> groups(c->nblk);  // slots of each group of each s block
+    std::vector<std::vector<std::vector<int>>> groups(c->nblk);  // slots of each group of each block
     bool twins = false;
     for (int X = 0; X < c->nblk; ++X) {
-        if (c->blk_L[X] != 0) continue;
         const std::vector<int>& sh = c->blk_shells[X];
         for (size_t s = 0; s < sh.size();) {
             std::vector<int> g{(int)s};
-            while ((int)g.size() < kGcMaxShells && s + g.size() < sh.size() &&
+            while (c->blk_L[X] == 0 && (int)g.size() < kGcMaxShells && s + g.size() < sh.size() &&
                    same_exponents(c, sh[s], sh[s + g.size()]))
                 g.push_back((int)(s + g.size()));
             twins = twins || g.size() > 1;
@@ -239,14 +240,13 @@ bool build_groups(hfg_ctx* c, double mq, double sig_cut, std::vector<double4>& g
     struct Entry {
         double key = 0.0;
         GcPairDev gp{};
-        std::vector<double4> p0, K;
+        std::vector<double4> p0, p1, K;
     };
     std::vector<int> pidx(kBlockFuncs * kBlockFuncs);
-    std::vector<double4> rest0, restK;  // a group pair's insignificant primitive pairs
+    std::vector<double4> rest0, rest1, restK;  // a group pair's insignificant primitive pairs
     for (int g = 0; g < nbp; ++g) {
         c->goff[g] = (int)c->gpairs.size();
         const int X = c->bp_X[g], Y = c->bp_Y[g];
-        if (c->blk_L[X] != 0 || c->blk_L[Y] != 0) continue;
         std::fill(pidx.begin(), pidx.end(), -1);
         for (int k = c->bp_off[g]; k < c->bp_off[g] + c->bp_cnt[g]; ++k)
             pidx[c->pairs[k].slot_a * kBlockFuncs + c->pairs[k].slot_b] = k;
@@ -295,12 +295,15 @@ bool build_groups(hfg_ctx* c, double mq, double sig_cut, std::vector<double4>& g
                         double P[3];
                         for (int d = 0; d < 3; ++d) P[d] = (ai * A[d] + bj * B[d]) / p;
                         (sig ? e.p0 : rest0).push_back(make_double4(p, P[0], P[1], P[2]));
+                        (sig ? e.p1 : rest1).push_back(make_double4(P[0] - A[0], P[1] - A[1], P[2] - A[2], 0.5 / p));
                         (sig ? e.K : restK).push_back(make_double4(K[0], K[1], K[2], K[3]));
                     }
                 e.gp.np = (int)e.p0.size();
                 e.p0.insert(e.p0.end(), rest0.begin(), rest0.end());
+                e.p1.insert(e.p1.end(), rest1.begin(), rest1.end());
                 e.K.insert(e.K.end(), restK.begin(), restK.end());
                 rest0.clear();
+                rest1.clear();
                 restK.clear();
                 ents.push_back(std::move(e));
             }
@@ -309,11 +312,13 @@ bool build_groups(hfg_ctx* c, double mq, double sig_cut, std::vector<double4>& g
         const size_t base = gp0.size(), cnt = ents.size();
         const size_t nfull = ents.empty() ? 0 : ents[0].p0.size();
         gp0.resize(base + nfull * cnt);
+        gp1.resize(gp0.size());
         gK.resize(gp0.size());
         for (size_t j = 0; j < cnt; ++j) {
             Entry& e = ents[j];
             for (size_t k = 0; k < nfull; ++k) {
                 gp0[base + k * cnt + j] = e.p0[k];
+                gp1[base + k * cnt + j] = e.p1[k];
                 gK[base + k * cnt + j] = e.K[k];
             }
             e.gp.off = (int)(base + j);
@@ -573,8 +578,8 @@ int setup_geometry(hfg_ctx* c) {
         HFG_CHECK(c, c->d_pp2.upload(q2, c->stream));
     }
     // shell groups of the s blocks and their group-pair lists
-    std::vector<double4> gp0, gK;
-    const bool gc_on = build_groups(c, mq, sig_cut, gp0, gK);
+    std::vector<double4> gp0, gp1, gK;
+    const bool gc_on = build_groups(c, mq, sig_cut, gp0, gp1, gK);
     std::vector<double> gcum_np, gcum_npnm, gcum_nm;  // per block pair prefixes over its group list
     if (gc_on) {
         HFG_CHECK(c, c->d_gpairs.upload(c->gpairs, c->stream));
@@ -582,8 +587,10 @@ int setup_geometry(hfg_ctx* c) {
         HFG_CHECK(c, c->d_goff.upload(c->goff, c->stream));
         HFG_CHECK(c, c->d_gcnt.upload(c->gcnt, c->stream));
         HFG_CHECK(c, c->d_gc_p0.upload(gp0, c->stream));
+        HFG_CHECK(c, c->d_gc_p1.upload(gp1, c->stream));
         HFG_CHECK(c, c->d_gc_K.upload(gK, c->stream));
-        c->gc_args = GcArgs{c->d_gpairs.p, c->d_gkey.p, c->d_goff.p, c->d_gcnt.p, c->d_gc_p0.p, c->d_gc_K.p};
+        c->gc_args =
+            GcArgs{c->d_gpairs.p, c->d_gkey.p, c->d_goff.p, c->d_gcnt.p, c->d_gc_p0.p, c->d_gc_p1.p, c->d_gc_K.p};
         gcum_np.assign(c->gpairs.size() + nbp, 0.0);
         gcum_npnm.assign(gcum_np.size(), 0.0);
         gcum_nm.assign(gcum_np.size(), 0.0);
@@ -651,27 +658,35 @@ int setup_geometry(hfg_ctx* c) {
                     class_cost[ci] += cost;
                 }
             }
-    // (ss|ss) on the group lists: ket count and cost of group row i of list P against list Q
-    // (model: per group primitive quartet the prefactor and Boys value of one primitive quartet plus
-    // a multiply, a compare and an add per member pair; per member shell quartet the digestion)
-    auto gc_row = [&](int P, int Q, int i, const FlopParts& fp) {
+    // A class on the group lists: ket count and cost of group row i of list P against list Q
+    // (model: per group primitive quartet the prefactor, Boys values and vertical recursion of one
+    // primitive quartet, plus per member pair its weight, cutoff and a multiply-add per
+    // accumulated value; per member shell quartet the horizontal recursion and digestion)
+    auto gc_row = [&](int P, int Q, int i, const ClassWork& cw, const FlopParts& fp) {
         int cnt = ket_count_host(&c->gkey[c->goff[Q]], c->gcnt[Q], c->gkey[c->goff[P] + i], c->tol);
         if (P == Q) cnt = std::min(cnt, i + 1);
         const GcPairDev& gb = c->gpairs[c->goff[P] + i];
         const int k0 = c->goff[Q] + Q;
-        const double cost = gb.np * (fp.per_prim * gcum_np[k0 + cnt] + 3.0 * gb.nmem * gcum_npnm[k0 + cnt]) +
+        auto ncum = [](int l) { return (l + 1) * (l + 2) * (l + 3) / 6; };
+        const double vals = (double)(ncum(cw.la + cw.lb) - ncum(cw.la - 1)) * (ncum(cw.lc + cw.ld) - ncum(cw.lc - 1));
+        const double cost = gb.np * (fp.per_prim * gcum_np[k0 + cnt] + (3.0 + 2.0 * vals) * gb.nmem * gcum_npnm[k0 + cnt]) +
                             fp.per_quartet * gb.nmem * gcum_nm[k0 + cnt];
         return std::make_pair(cnt, cost);
     };
-    double gc_cost = 0;  // the (ss|ss) class's cost on the group lists
-    if (gc_on) {
-        const FlopParts fp = model_flops(0, 0, 0, 0, c->pure, false);
+    // each class's cost on the group lists (the classes with a group kernel)
+    std::vector<double> gc_cost(class_cost.size(), 0.0);
+    if (gc_on)
         for (const auto& M : c->mols)
             for (int P = M.bp0; P < M.bp0 + M.nbp; ++P)
-                for (int Q = M.bp0; Q <= P; ++Q)
-                    if (c->gcnt[P] && c->gcnt[Q])
-                        for (int i = 0; i < c->gcnt[P]; ++i) gc_cost += gc_row(P, Q, i, fp).second;
-    }
+                for (int Q = M.bp0; Q <= P; ++Q) {
+                    if (!c->bp_cnt[P] || !c->bp_cnt[Q] || !c->gcnt[P] || !c->gcnt[Q]) continue;
+                    const int A = c->bp_X[P], B = c->bp_Y[P], C = c->bp_X[Q], D = c->bp_Y[Q];
+                    const int ci = class_index(c->blk_L[A], c->blk_L[B], c->blk_L[C], c->blk_L[D]);
+                    const ClassWork& cw = c->classes[ci];
+                    if (!jk_gc_available(cw.la, cw.lb, cw.lc, cw.ld, c->pure)) continue;
+                    const FlopParts fp = model_flops(cw.la, cw.lb, cw.lc, cw.ld, c->pure, false);
+                    for (int i = 0; i < c->gcnt[P]; ++i) gc_cost[ci] += gc_row(P, Q, i, cw, fp).second;
+                }
     const double split = c->opt.work_split > 0 ? (double)c->opt.work_split : kDefaultWorkSplit;
     const double target = std::max(total_cost / (256.0 * 6.0 * split), 1.0);
     c->slab_doubles = 0;
@@ -727,7 +742,8 @@ int setup_geometry(hfg_ctx* c) {
                 const double tgt = std::max(std::min(target, class_cost[ci] / (256.0 * split)), 1.0);
                 // (ss|ss) on the group lists: the J/K items come from those, the per-shell items
                 // below serve the gradient only
-                const bool on_groups = gc_on && cw.la + cw.lb + cw.lc + cw.ld == 0 && c->gcnt[P] && c->gcnt[Q];
+                const bool on_groups =
+                    gc_on && jk_gc_available(cw.la, cw.lb, cw.lc, cw.ld, c->pure) && c->gcnt[P] && c->gcnt[Q];
                 WorkItem w{};
                 w.split = 1;
                 w.P = P;
@@ -778,9 +794,9 @@ int setup_geometry(hfg_ctx* c) {
                     cw.gc_items.push_back(v);
                 };
                 // as many items as the per-shell lists would give the class (its items are cheaper)
-                const double tgt_gc = std::max(tgt * gc_cost / std::max(class_cost[ci], 1.0), 1.0);
+                const double tgt_gc = std::max(tgt * gc_cost[ci] / std::max(class_cost[ci], 1.0), 1.0);
                 split_rows(c->gcnt[P], tgt_gc, [&](int i) {
-                    const std::pair<int, double> rc = gc_row(P, Q, i, fp);
+                    const std::pair<int, double> rc = gc_row(P, Q, i, cw, fp);
                     cw.gc_flops += rc.second;
                     return rc;
                 }, push_gc);
@@ -1270,7 +1286,7 @@ void layout_molecules(hfg_ctx* c) {
 
 hipError_t hfg::launch_jk_class(hfg_ctx* c, const ClassWork& cw, const JkArgs& a, LaunchShape sh, hipStream_t st) {
     const int mixed = c->opt.precision == HFG_MIXED ? 1 : 0;
-    if (cw.gc) return launch_jk_gc_ssss(a, c->gc_args, mixed, (int)cw.gc_items.size(), sh, st);
+    if (cw.gc) return launch_jk_gc(cw.la, cw.lb, cw.lc, cw.ld, c->pure, mixed, a, c->gc_args, (int)cw.gc_items.size(), sh, st);
     if (cw.coop) return launch_jk_coop(a, cw.coop_args, cw.coop_lds[mixed], mixed, (int)cw.items.size(), sh, st);
     return launch_jk(cw.la, cw.lb, cw.lc, cw.ld, c->pure, mixed, a, (int)cw.items.size(), sh, st);
 }
@@ -1809,6 +1825,7 @@ void hfg_destroy(hfg_ctx* c) {
     c->d_goff.release();
     c->d_gcnt.release();
     c->d_gc_p0.release();
+    c->d_gc_p1.release();
     c->d_gc_K.release();
     c->d_gJ_off.release();
     c->d_gK_off.release();

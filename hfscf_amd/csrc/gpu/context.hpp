@@ -209,7 +209,7 @@ struct hfg_ctx {
     DevBuf<hfg::GcPairDev> d_gpairs;
     DevBuf<double> d_gkey;
     DevBuf<int> d_goff, d_gcnt;
-    DevBuf<double4> d_gc_p0, d_gc_K;
+    DevBuf<double4> d_gc_p0, d_gc_p1, d_gc_K;
     hfg::GcArgs gc_args{};  // device pointers of the group tables
     // launch graphs of the J/K build (keyed by precision and density screening)
     std::vector<hfg::JkGraph> graphs;

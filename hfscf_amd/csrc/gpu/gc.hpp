@@ -1,5 +1,5 @@
-// Shells that share exponents: pair groups and the (ss|ss) J/K kernel that evaluates each
-// group's primitive quartets once for every member shell quartet.
+// Shells that share exponents: pair groups and the J/K kernels that evaluate each group
+// quartet's primitive quartets once for every member shell quartet.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -8,9 +8,14 @@
 
 namespace hfg {
 
-// Most shells of one atom and angular momentum with identical exponents that form one group.
+// Most s shells of one atom with identical exponents that form one group (the shells of higher
+// angular momentum stay single-shell groups).
 constexpr int kGcMaxShells = 2;
 constexpr int kGcMaxMembers = kGcMaxShells * kGcMaxShells;
+// Most member shell pairs of a group pair of blocks with angular momenta (l1, l2).
+__host__ __device__ constexpr int gc_members(int l1, int l2) {
+    return (l1 == 0 ? kGcMaxShells : 1) * (l2 == 0 ? kGcMaxShells : 1);
+}
 // Lanes per group quartet (WorkItem::split): enough that one lane evaluates about kGcLanePrims
 // primitive quartets, at most kGcMaxSplit.
 constexpr double kGcLanePrims = 512.0;
@@ -39,11 +44,16 @@ struct GcArgs {
     const int* goff;          // first group of each block pair's list
     const int* gcnt;          // groups in each block pair's list
     const double4* gp0;       // per group primitive pair: {p, P}
+    const double4* gp1;       // per group primitive pair: {P - A, 1/(2p)}
     const double4* gK;        // per group primitive pair: K of members 0..3
 };
 
-// The (ss|ss) class on the group lists (mixed != 0: the FP32 primitive loop of the mixed builds).
-hipError_t launch_jk_gc_ssss(const JkArgs& a, const GcArgs& g, int mixed, int nitems, LaunchShape sh,
-                             hipStream_t s);
+// Whether class (la lb|lc ld) has a kernel on the group lists for this basis type.
+bool jk_gc_available(int la, int lb, int lc, int ld, bool pure);
+
+// The J/K launch of a class on the group lists (mixed != 0: the FP32 primitive loops of the mixed
+// builds); the class must be jk_gc_available.
+hipError_t launch_jk_gc(int la, int lb, int lc, int ld, bool pure, int mixed, const JkArgs& a, const GcArgs& g,
+                        int nitems, LaunchShape sh, hipStream_t s);
 
 }  // namespace hfg

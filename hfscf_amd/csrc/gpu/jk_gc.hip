@@ -14,6 +14,7 @@
 
 #include "boys32.hpp"
 #include "gc.hpp"
+#include "jk_gc_impl.hpp"
 #include "jk_impl.hpp"
 
 namespace hfg {
@@ -195,18 +196,49 @@ __global__ __launch_bounds__(256) void jk_gc_ssss_kernel(JkArgs a, GcArgs g) {
     }
 }
 
-hipError_t launch_jk_gc_ssss(const JkArgs& a, const GcArgs& g, int mixed, int nitems, LaunchShape sh,
-                             hipStream_t s) {
+#define HFG_GC_CLASS(A, B, C, D, P)                                                                    \
+    template <>                                                                                        \
+    hipError_t launch_jk_gc_class64<A, B, C, D, P>(const JkArgs&, const GcArgs&, int, LaunchShape, hipStream_t); \
+    template <>                                                                                        \
+    hipError_t launch_jk_gc_class32<A, B, C, D, P>(const JkArgs&, const GcArgs&, int, LaunchShape, hipStream_t);
+#define HFG_GC_CLASSES(X) X(1, 0, 0, 0)
+#define HFG_GC_DECL(A, B, C, D) HFG_GC_CLASS(A, B, C, D, true) HFG_GC_CLASS(A, B, C, D, false)
+HFG_GC_CLASSES(HFG_GC_DECL)
+
+bool jk_gc_available(int la, int lb, int lc, int ld, bool) {
+    const int key = la * 1000 + lb * 100 + lc * 10 + ld;
+#define HFG_GC_KEY(A, B, C, D) if (key == A * 1000 + B * 100 + C * 10 + D) return true;
+    HFG_GC_CLASSES(HFG_GC_KEY)
+#undef HFG_GC_KEY
+    return key == 0;
+}
+
+hipError_t launch_jk_gc(int la, int lb, int lc, int ld, bool pure, int mixed, const JkArgs& a, const GcArgs& g,
+                        int nitems, LaunchShape sh, hipStream_t s) {
     if (nitems == 0) return hipSuccess;
-    JkArgs b = a;
-    b.nitems = nitems;
-    const int block = sh.block > 0 ? sh.block : 256;
-    const int grid = sh.grid > 0 ? std::min(sh.grid, nitems) : nitems;
-    if (mixed)
-        hipLaunchKernelGGL(jk_gc_ssss_kernel<float>, dim3(grid), dim3(block), 0, s, b, g);
-    else
-        hipLaunchKernelGGL(jk_gc_ssss_kernel<double>, dim3(grid), dim3(block), 0, s, b, g);
-    return hipGetLastError();
+    const int key = la * 1000 + lb * 100 + lc * 10 + ld;
+    if (key == 0) {
+        JkArgs b = a;
+        b.nitems = nitems;
+        const int block = sh.block > 0 ? sh.block : 256;
+        const int grid = sh.grid > 0 ? std::min(sh.grid, nitems) : nitems;
+        if (mixed)
+            hipLaunchKernelGGL(jk_gc_ssss_kernel<float>, dim3(grid), dim3(block), 0, s, b, g);
+        else
+            hipLaunchKernelGGL(jk_gc_ssss_kernel<double>, dim3(grid), dim3(block), 0, s, b, g);
+        return hipGetLastError();
+    }
+#define HFG_GC_CALL(A, B, C, D)                                                                           \
+    if (key == A * 1000 + B * 100 + C * 10 + D) {                                                         \
+        if (pure)                                                                                         \
+            return mixed ? launch_jk_gc_class32<A, B, C, D, true>(a, g, nitems, sh, s)                   \
+                         : launch_jk_gc_class64<A, B, C, D, true>(a, g, nitems, sh, s);                  \
+        return mixed ? launch_jk_gc_class32<A, B, C, D, false>(a, g, nitems, sh, s)                      \
+                     : launch_jk_gc_class64<A, B, C, D, false>(a, g, nitems, sh, s);                     \
+    }
+    HFG_GC_CLASSES(HFG_GC_CALL)
+#undef HFG_GC_CALL
+    return hipErrorInvalidValue;
 }
 
 }  // namespace hfg
