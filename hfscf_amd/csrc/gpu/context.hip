@@ -45,16 +45,24 @@ hipStream_t shared_scratch_stream(int device) {
     return streams[device];
 }
 
-// Streams of the class launches (J/K, or the gradient when grad): the classes that spill run on
-// the device's shared stream (-1); the others are balanced over the side streams, heaviest
-// first onto the least loaded one, by the weights w (the FLOP model at set-up, the measured
-// kernel times after autotuning).
+// Streams and launch order of the class launches (J/K, or the gradient when grad): the classes
+// that spill run on the device's shared stream (-1); the others are balanced over the side
+// streams, onto the least loaded one, by the weights w (the FLOP model at set-up, the measured
+// kernel times after autotuning). The launch order is the cooperative-kernel classes first (one
+// quartet per workgroup at a time: few workgroups on long serial chains, whose latency then
+// hides under the wide launches instead of trailing the build), then heaviest first.
 void assign_streams(hfg_ctx* c, bool grad, const std::vector<double>& w) {
     std::vector<int>& map = grad ? c->grad_stream : c->class_stream;
     map.assign(c->classes.size(), 0);
     std::vector<size_t> ord(c->classes.size());
     std::iota(ord.begin(), ord.end(), 0);
-    std::stable_sort(ord.begin(), ord.end(), [&](size_t x, size_t y) { return w[x] > w[y]; });
+    auto coop = [&](size_t k) { return grad ? c->classes[k].gcoop : c->classes[k].coop; };
+    std::stable_sort(ord.begin(), ord.end(), [&](size_t x, size_t y) {
+        if (coop(x) != coop(y)) return coop(x);
+        return w[x] > w[y];
+    });
+    std::vector<int>& order = grad ? c->grad_order : c->class_order;
+    order.assign(ord.begin(), ord.end());
     double load[kSideStreams] = {0};
     const int nside = std::max(1, std::min(c->nside, kSideStreams));
     for (size_t k : ord) {

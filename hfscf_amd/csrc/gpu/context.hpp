@@ -142,6 +142,7 @@ struct hfg_ctx {
     hipEvent_t ev_heavy_join = nullptr;
     std::vector<int> class_stream;  // side stream of each class's J/K launch (-1: the shared stream)
     std::vector<int> grad_stream;   // the same for the gradient launches
+    std::vector<int> class_order, grad_order;  // launch order of the classes (J/K, gradient)
     hfg_options opt{};
     std::string err;
     bool have_basis = false, have_geom = false;
@@ -256,8 +257,9 @@ inline int shared_class_count(const hfg_ctx* c, bool grad) {
 }
 
 // Forks `main` to the streams of the selected classes, calls fn(class, stream, counter index)
-// for each, and joins them back into `main` (the shared stream only with join_shared). Counter indices: shared-stream classes 0.., then the
-// side-stream classes, in launch order.
+// for each in the launch order, and joins them back into `main` (the shared stream only with
+// join_shared). Counter indices: shared-stream classes 0.., then the side-stream classes, in
+// launch order.
 template <typename F>
 inline hipError_t for_each_class_concurrent(hfg_ctx* c, hipStream_t main, F&& fn, bool grad = false,
                                             ClassSet set = ClassSet::All, bool join_shared = true) {
@@ -271,15 +273,18 @@ inline hipError_t for_each_class_concurrent(hfg_ctx* c, hipStream_t main, F&& fn
     if (shared && (e = hipStreamWaitEvent(c->heavy, c->ev_fork, 0)) != hipSuccess) return e;
     // the shared-stream classes first: they are few workgroups of long latency, and launched
     // after the side streams' persistent grids they would wait for compute units to drain
+    const std::vector<int>& order = grad ? c->grad_order : c->class_order;
     int q = 0;
     for (int pass = 0; pass < 2; ++pass)
-        for (size_t i = 0; i < c->classes.size(); ++i)
+        for (size_t n = 0; n < c->classes.size(); ++n) {
+            const size_t i = order.size() == c->classes.size() ? (size_t)order[n] : n;
             if ((streams[i] < 0) == (pass == 0)) {
                 const int qi = q++;
                 if ((pass == 0 && !shared) || (pass == 1 && !side)) continue;
                 if ((e = fn(c->classes[i], streams[i] < 0 ? c->heavy : c->side[streams[i]], qi)) != hipSuccess)
                     return e;
             }
+        }
     if (side)
         for (int k = 0; k < c->nside; ++k) {
             if ((e = hipEventRecord(c->ev_join[k], c->side[k])) != hipSuccess) return e;
