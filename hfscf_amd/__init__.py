@@ -100,6 +100,8 @@ def hfgpu():
         lib.hfg_launch_shapes.argtypes = [C.c_void_p, C.c_int, _ip, _ip, _ip, _ip, _ip]
         lib.hfg_energy.argtypes = [C.c_void_p, _dp, _dp]
         lib.hfg_energy_backward.argtypes = [C.c_void_p, _dp, _dp]
+        lib.hfg_oneint.argtypes = [C.c_void_p, _dp, _dp, _dp]
+        lib.hfg_oneint_gradient.argtypes = [C.c_void_p, _dp, _dp, _dp]
         lib.hfg_eri_shell_quartet.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, _dp]
         lib.hfg_schwarz.argtypes = [C.c_void_p, _dp]
         lib.hfg_copy_calibration.argtypes = [C.c_int, C.c_longlong, C.c_int, _dp]
@@ -473,6 +475,21 @@ class GpuContext:
         g = np.zeros((self.natom, 3))
         D = np.ascontiguousarray(D, dtype=np.float64)
         self._check(hfgpu().hfg_energy_backward(self._h, _ptr(D), _ptr(g)))
+        return g
+
+    def oneint(self):
+        """S, T, V of the molecule from the GPU one-electron kernels (single-molecule contexts)."""
+        n = self.nbf or 1  # a batch context is refused by the library
+        S, T, V = np.zeros((n, n)), np.zeros((n, n)), np.zeros((n, n))
+        self._check(hfgpu().hfg_oneint(self._h, _ptr(S), _ptr(T), _ptr(V)))
+        return S, T, V
+
+    def oneint_gradient(self, D, Q):
+        """2 D.d(T+V) - 2 Q.dS (natom x 3) from the GPU one-electron kernels."""
+        g = np.zeros((self.natom, 3))
+        D = np.ascontiguousarray(D, dtype=np.float64)
+        Q = np.ascontiguousarray(Q, dtype=np.float64)
+        self._check(hfgpu().hfg_oneint_gradient(self._h, _ptr(D), _ptr(Q), _ptr(g)))
         return g
 
     def eri_block(self, s1, s2, s3, s4):

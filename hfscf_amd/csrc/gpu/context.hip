@@ -1394,7 +1394,6 @@ int hfg_set_basis(hfg_ctx* c, int nshell, const int* L, const int* atom_of_shell
 }
 
 int hfg_set_geometry(hfg_ctx* c, int natom, const double* xyz_bohr, const int* Z) {
-    (void)Z;
     if (!c || !c->have_basis || natom <= 0 || !xyz_bohr) return fail(c, HFG_ERR_ARG, "hfg_set_geometry: bad arguments");
     if (c->batch) return fail(c, HFG_ERR_ARG, "hfg_set_geometry: the context holds a batch of molecules");
     for (int s = 0; s < c->nshell; ++s)
@@ -1404,6 +1403,9 @@ int hfg_set_geometry(hfg_ctx* c, int natom, const double* xyz_bohr, const int* Z
     c->mols[0].atom0 = 0;
     c->mols[0].natom = natom;
     c->xyz.assign(xyz_bohr, xyz_bohr + 3 * natom);
+    c->Zn.clear();
+    if (Z)
+        for (int a = 0; a < natom; ++a) c->Zn.push_back((double)Z[a]);
     layout_molecules(c);
     c->have_geom = false;
     int r = setup_geometry(c);
@@ -1833,6 +1835,13 @@ void hfg_destroy(hfg_ctx* c) {
     c->d_goff.release();
     c->d_gcnt.release();
     c->d_gc_p0.release();
+    c->d_one_sh.release();
+    c->d_one_pairs.release();
+    c->d_one_alpha.release();
+    c->d_one_coef.release();
+    c->d_one_xyz.release();
+    c->d_one_Z.release();
+    c->d_one_buf.release();
     c->d_gc_p1.release();
     c->d_gc_K.release();
     c->d_gJ_off.release();

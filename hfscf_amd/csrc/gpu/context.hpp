@@ -13,6 +13,7 @@
 #include "gc.hpp"
 #include "kernels.hpp"
 #include "launch_bounds.hpp"
+#include "oneint.hpp"
 
 namespace hfg {
 
@@ -159,6 +160,7 @@ struct hfg_ctx {
     // geometry (concatenated)
     int natom = 0;
     std::vector<double> xyz;
+    std::vector<double> Zn;  // nuclear charges of a single molecule (hfg_set_geometry), for hfg_oneint
     // blocks
     int nblk = 0;
     std::vector<int> blk_L, blk_nf, blk_func_off, blk_func_ao, blk_mol, shell_blk, shell_slot;
@@ -212,6 +214,10 @@ struct hfg_ctx {
     DevBuf<int> d_goff, d_gcnt;
     DevBuf<double4> d_gc_p0, d_gc_p1, d_gc_K;
     hfg::GcArgs gc_args{};  // device pointers of the group tables
+    // one-electron integrals (hfg_oneint, hfg_oneint_gradient)
+    DevBuf<hfg::OneShell> d_one_sh;
+    DevBuf<int2> d_one_pairs;
+    DevBuf<double> d_one_alpha, d_one_coef, d_one_xyz, d_one_Z, d_one_buf;
     // launch graphs of the J/K build (keyed by precision and density screening)
     std::vector<hfg::JkGraph> graphs;
     bool class_timed = false;  // the last J/K build bracketed each class launch with events

@@ -183,6 +183,18 @@ int hfg_batch_commit(hfg_ctx* ctx);
 /* Molecule count, and per molecule its basis size, matrix offset and atom count (any may be NULL). */
 int hfg_batch_layout(const hfg_ctx* ctx, int* nmol, int* nbf, long long* moff, int* natom);
 
+/* One-electron integrals of the context's single molecule on the GPU: S, T and V (nbf x nbf
+ * row-major host arrays in the output basis; any may be NULL), with the nuclear charges given to
+ * hfg_set_geometry. Replaces: OSOverlap / OSNuclear compute_contracted_shell as called by
+ * hf_solver::create_one_electron_hamiltonian (hfscf: src/scfsolver/hfscf_hf_hamiltonian.cpp:14-38;
+ * src/integrals/hfscf_osoverlap.cpp:10, hfscf_osnuclear.cpp:17-83). */
+int hfg_oneint(hfg_ctx* ctx, double* S, double* T, double* V);
+/* grad (natom x 3, host, added into) += 2 sum_ij D_ij d(T+V)_ij/dX - 2 sum_ij Q_ij dS_ij/dX on the
+ * GPU, D the half density and Q = D F D. Replaces: the dTdX, dVndX and dSdX terms of
+ * calc_scf_gradient_rhf (hfscf: src/gradient/hfscf_grad_rhf.cpp:143-167; hfscf_osoverlapderiv1.cpp:8,
+ * hfscf_oskineticderiv1.cpp:8, hfscf_osnuclearderiv1.cpp:15). */
+int hfg_oneint_gradient(hfg_ctx* ctx, const double* D, const double* Q, double* grad);
+
 /* Autotuner: times every class launch alone over candidate launch shapes (threads per
  * workgroup x persistent workgroups) on the density of the most recent build and keeps the
  * fastest per class. what: 1 = J/K, 2 = gradient, 3 = both. */
