@@ -107,6 +107,14 @@ class GpuJK final : public JKBuilder {
         for (size_t k = 0; k < g.size(); ++k) grad[k] += g[k];
     }
     void set_final(bool final_build) override { final_ = final_build; }
+    bool oneint(double* S, double* T, double* V) override {
+        check(hfg_oneint(ctx_, S, T, V));
+        return true;
+    }
+    bool oneint_gradient(const double* D, const double* Q, double* grad) override {
+        check(hfg_oneint_gradient(ctx_, D, Q, grad));
+        return true;
+    }
 
    private:
     void check(int r) {

@@ -93,8 +93,13 @@ void hf_solver::init_data() {
     const size_t nn = (size_t)n_ * n_;
     iteration_ = 0;
     energies_.clear();
-    overlap_kinetic(mol_, S_, T_);
-    nuclear_attraction(mol_, V_);
+    S_.resize(nn);
+    T_.resize(nn);
+    V_.resize(nn);
+    if (!jk_ || !jk_->oneint(S_.data(), T_.data(), V_.data())) {  // the GPU backend computes them on the device
+        overlap_kinetic(mol_, S_, T_);
+        nuclear_attraction(mol_, V_);
+    }
     H_.resize(nn);
     for (size_t k = 0; k < nn; ++k) H_[k] = T_[k] + V_[k];
     if (on_device()) return;  // X, the guess and every later step are formed on the GPU
@@ -245,7 +250,7 @@ std::vector<double> calc_scf_gradient_rhf(const Molecule& mol, const double* D, 
     gemm(n, DF.data(), false, D, false, Q.data());
     std::vector<double> g((size_t)mol.natom() * 3, 0.0);
     jk->gradient_2e(D, g.data());
-    oneint_gradient(mol, D, Q.data(), g.data());
+    if (!jk->oneint_gradient(D, Q.data(), g.data())) oneint_gradient(mol, D, Q.data(), g.data());
     nuclear_repulsion_gradient(mol, g.data());
     return g;
 }

@@ -36,6 +36,17 @@ struct JKBuilder {
         return false;
     }
     virtual int device() const { return -1; }
+    // One-electron integrals S, T, V (n x n) from the backend's device; false when it has none and
+    // the host layer computes them.
+    virtual bool oneint(double* S, double* T, double* V) {
+        (void)S, (void)T, (void)V;
+        return false;
+    }
+    // grad += 2 D.d(T+V) - 2 Q.dS from the backend's device; false when it has none.
+    virtual bool oneint_gradient(const double* D, const double* Q, double* grad) {
+        (void)D, (void)Q, (void)grad;
+        return false;
+    }
 };
 
 struct ScfOptions {
