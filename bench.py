@@ -243,6 +243,43 @@ def scf_figures(mol, scf64, g64):
     return out
 
 
+def oneint_figures(mol, D, F):
+    """One-electron integrals (S, T, V) and their gradient terms on the GPU against the host layer
+    (OpenMP, the same formulas), at the workload's converged density; Q = D F D."""
+    ctx = H.GpuContext(mol)
+    Q = D @ F @ D
+    out = {}
+
+    def best(fn, reps):
+        ts = []
+        for _ in range(reps):
+            t = time.perf_counter()
+            r = fn()
+            ts.append(time.perf_counter() - t)
+        return min(ts), r
+
+    ctx.oneint()
+    ctx.oneint_gradient(D, Q)  # warm-up (code objects, buffers)
+    t_g, (S, T, V) = best(ctx.oneint, 5)
+    t_gg, gg = best(lambda: ctx.oneint_gradient(D, Q), 5)
+    ctx.close()
+    t_h, (Sh, Th, Vh) = best(mol.oneint, 2)
+    Z, X = mol.geometry()
+    t_hg, gh = best(lambda: mol.oneint_gradient(D, Q), 2)
+    # the host gradient terms include the nuclear repulsion gradient, the device ones do not
+    enn = mol.oneint_gradient(np.zeros_like(D), np.zeros_like(Q))
+    out["oneint_gpu_ms"] = 1e3 * t_g
+    out["oneint_host_ms"] = 1e3 * t_h
+    out["oneint_gradient_gpu_ms"] = 1e3 * t_gg
+    out["oneint_gradient_host_ms"] = 1e3 * t_hg
+    out["oneint_max_abs_diff_gpu_vs_host"] = float(max(np.abs(S - Sh).max(), np.abs(T - Th).max(), np.abs(V - Vh).max()))
+    out["oneint_gradient_max_abs_diff_gpu_vs_host"] = float(np.abs(gg + enn - gh).max())
+    out["oneint_host_threads"] = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    log(f"one-electron: S,T,V {1e3 * t_g:.2f} ms GPU vs {1e3 * t_h:.1f} ms host; gradient terms "
+        f"{1e3 * t_gg:.2f} ms GPU vs {1e3 * t_hg:.1f} ms host")
+    return out
+
+
 def batch_figures(count, sample, seed=11):
     """Batching mode on the seeded (H2O)n clusters (n in 1..10, 6-31G*): all molecules in one
     lockstep SCF with one batched build per iteration (HIP graphs, autotuned launch shapes), then
@@ -388,6 +425,8 @@ def main():
             if not args.no_scf:
                 g64 = H.rhf_gradient(mol, scf["D"], scf["F"])
                 ex.update(scf_figures(mol, scf, g64))
+        if not args.no_scf:
+            ex.update(oneint_figures(mol, scf["D"], scf["F"]))
         if args.batch_molecules > 0:
             ex.update(batch_figures(args.batch_molecules, args.solo_sample))
         out["extra"] = ex

@@ -141,8 +141,42 @@ def hfscf():
         lib.hfscf_batch_rhf.argtypes = [C.c_int, C.POINTER(C.c_void_p), C.POINTER(ScfOptions), C.POINTER(ScfResult),
                                         _dp, _dp, _dp, C.POINTER(BatchStats)]
         lib.hfscf_last_error.restype = C.c_char_p
+        lib.hfscf_input_read.argtypes = [C.c_char_p]
+        lib.hfscf_input_read.restype = C.c_void_p
+        lib.hfscf_input_destroy.argtypes = [C.c_void_p]
+        lib.hfscf_input_info.argtypes = [C.c_void_p, _ip, _ip, _ip]
+        lib.hfscf_input_atoms.argtypes = [C.c_void_p, _ip, _dp]
+        lib.hfscf_input_setting.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_char_p), C.POINTER(C.c_char_p)]
+        lib.hfscf_input_basis_file.argtypes = [C.c_void_p]
+        lib.hfscf_input_basis_file.restype = C.c_char_p
+        lib.hfscf_input_comment.argtypes = [C.c_void_p]
+        lib.hfscf_input_comment.restype = C.c_char_p
         lib._typed = True
     return lib
+
+
+def read_input(path):
+    """An hfscf .dat input file: comment, keywords (validated against hfscf's table), atoms (Z,
+    bohr), charge, multiplicity and the basis file name (hfscf_input_read)."""
+    lib = hfscf()
+    h = lib.hfscf_input_read(str(path).encode())
+    if not h:
+        raise ValueError(lib.hfscf_last_error().decode())
+    try:
+        n, q, m = C.c_int(), C.c_int(), C.c_int()
+        lib.hfscf_input_info(h, C.byref(n), C.byref(q), C.byref(m))
+        Z = np.zeros(n.value, dtype=np.int32)
+        X = np.zeros(3 * n.value)
+        lib.hfscf_input_atoms(h, Z.ctypes.data_as(_ip), _ptr(X))
+        settings = {}
+        k, v = C.c_char_p(), C.c_char_p()
+        for i in range(lib.hfscf_input_setting(h, 0, None, None)):
+            lib.hfscf_input_setting(h, i, C.byref(k), C.byref(v))
+            settings[k.value.decode()] = v.value.decode()
+        return dict(comment=lib.hfscf_input_comment(h).decode(), settings=settings, Z=Z, xyz=X.reshape(-1, 3),
+                    charge=q.value, multiplicity=m.value, basis_file=lib.hfscf_input_basis_file(h).decode())
+    finally:
+        lib.hfscf_input_destroy(h)
 
 
 def _ptr(a, dtype=np.float64):
@@ -180,6 +214,15 @@ class Molecule:
         Z = np.ascontiguousarray(Z, dtype=np.int32)
         x = np.ascontiguousarray(xyz_bohr, dtype=np.float64).reshape(-1)
         return cls(hfscf().hfscf_mol_create(len(Z), Z.ctypes.data_as(_ip), _ptr(x), basis_path(basis).encode(), pure))
+
+    @classmethod
+    def from_input(cls, path):
+        """The molecule of an hfscf .dat input (its atoms, basis_set and use_puream)."""
+        inp = read_input(path)
+        if "basis_set" not in inp["settings"]:
+            raise ValueError(f"{path}: no basis_set keyword")
+        pure = {"true": 1, "false": 0}.get(inp["settings"].get("use_puream"), -1)
+        return cls.from_arrays(inp["Z"], inp["xyz"], inp["settings"]["basis_set"], pure)
 
     def __del__(self):
         if getattr(self, "_h", None):

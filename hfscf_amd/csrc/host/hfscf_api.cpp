@@ -10,6 +10,7 @@
 #include "../../../include/hfgpu.h"
 #include "../../../include/hfscf.h"
 #include "basis.hpp"
+#include "input.hpp"
 #include "linalg.hpp"
 #include "oneint.hpp"
 #include "scf.hpp"
@@ -469,6 +470,56 @@ int hfscf_mol_set_geometry(hfscf_mol* m, const double* xyz_bohr) {
 }
 
 double hfscf_mol_enuc(const hfscf_mol* m) { return m ? m->mol.nuclear_repulsion() : 0.0; }
+
+struct hfscf_input {
+    hfscf::InputFile in;
+    std::vector<std::pair<std::string, std::string>> kv;  // settings in keyword order
+};
+
+hfscf_input* hfscf_input_read(const char* path) {
+    if (!path) {
+        set_err("null path");
+        return nullptr;
+    }
+    auto h = std::make_unique<hfscf_input>();
+    std::string err;
+    if (!hfscf::read_input(path, h->in, err)) {
+        set_err(err);
+        return nullptr;
+    }
+    for (const auto& e : h->in.settings) h->kv.emplace_back(e.first, e.second);
+    return h.release();
+}
+
+void hfscf_input_destroy(hfscf_input* in) { delete in; }
+
+int hfscf_input_info(const hfscf_input* in, int* natom, int* charge, int* multiplicity) {
+    if (!in) return set_err("null input");
+    if (natom) *natom = (int)in->in.Z.size();
+    if (charge) *charge = in->in.charge;
+    if (multiplicity) *multiplicity = in->in.multiplicity;
+    return 0;
+}
+
+int hfscf_input_atoms(const hfscf_input* in, int* Z, double* xyz_bohr) {
+    if (!in) return set_err("null input");
+    if (Z) std::copy(in->in.Z.begin(), in->in.Z.end(), Z);
+    if (xyz_bohr) std::copy(in->in.xyz.begin(), in->in.xyz.end(), xyz_bohr);
+    return 0;
+}
+
+int hfscf_input_setting(const hfscf_input* in, int index, const char** keyword, const char** value) {
+    if (!in) return set_err("null input");
+    const int n = (int)in->kv.size();
+    if (index >= 0 && index < n) {
+        if (keyword) *keyword = in->kv[index].first.c_str();
+        if (value) *value = in->kv[index].second.c_str();
+    }
+    return n;
+}
+
+const char* hfscf_input_basis_file(const hfscf_input* in) { return in ? in->in.basis_file.c_str() : ""; }
+const char* hfscf_input_comment(const hfscf_input* in) { return in ? in->in.comment.c_str() : ""; }
 
 int hfscf_oneint(const hfscf_mol* m, double* S, double* T, double* V) {
     if (!m) return set_err("null molecule");

@@ -122,6 +122,25 @@ typedef struct hfscf_batch_stats {
 int hfscf_batch_rhf(int nmol, hfscf_mol* const* mols, const hfscf_scf_options* o, hfscf_scf_result* res,
                     double* D_all, double* F_all, double* grad_all, hfscf_batch_stats* stats);
 
+/* hfscf .dat input files: a comment line starting with '#', "keyword = value" lines checked
+ * against hfscf's keyword table (names, integer / real / free-text kinds, accepted values), and a
+ * { } block of atoms "symbol-or-Z x y z" in bohr, or angstrom with units = angstrom (dummy atoms X
+ * dropped, at most 100 lines). charge and multiplicity are read as hfscf reads them. A z-matrix
+ * block is refused. NULL on an error, with hfscf's message in hfscf_last_error().
+ * Replaces: MOLEC::Molecule::init_molecule and MOLPARSE (hfscf: src/molecule/hfscf_molecule.cpp:21-200,
+ * src/molecule/hfscf_parser.hpp:48-151, src/molecule/hfscf_keywords.hpp:14-72). */
+typedef struct hfscf_input hfscf_input;
+hfscf_input* hfscf_input_read(const char* path);
+void hfscf_input_destroy(hfscf_input* in);
+int hfscf_input_info(const hfscf_input* in, int* natom, int* charge, int* multiplicity);
+int hfscf_input_atoms(const hfscf_input* in, int* Z, double* xyz_bohr);
+/* Number of keywords given; with 0 <= index < that number, *keyword and *value receive the
+ * index-th (alphabetical order; the strings live as long as the input). */
+int hfscf_input_setting(const hfscf_input* in, int index, const char** keyword, const char** value);
+/* basis_set as hfscf's file name: lowercase, '*' -> "_st_", ".gbs" (hfscf_parser.hpp:134-151); "" if absent. */
+const char* hfscf_input_basis_file(const hfscf_input* in);
+const char* hfscf_input_comment(const hfscf_input* in);
+
 const char* hfscf_last_error(void);
 
 #ifdef __cplusplus
