@@ -101,6 +101,7 @@ def hfgpu():
         lib.hfg_energy.argtypes = [C.c_void_p, _dp, _dp]
         lib.hfg_energy_backward.argtypes = [C.c_void_p, _dp, _dp]
         lib.hfg_oneint.argtypes = [C.c_void_p, _dp, _dp, _dp]
+        lib.hfg_eri_vector.argtypes = [C.c_void_p, _dp, C.c_longlong]
         lib.hfg_oneint_gradient.argtypes = [C.c_void_p, _dp, _dp, _dp]
         lib.hfg_eri_shell_quartet.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, _dp]
         lib.hfg_schwarz.argtypes = [C.c_void_p, _dp]
@@ -534,6 +535,13 @@ class GpuContext:
         Q = np.ascontiguousarray(Q, dtype=np.float64)
         self._check(hfgpu().hfg_oneint_gradient(self._h, _ptr(D), _ptr(Q), _ptr(g)))
         return g
+
+    def eri_vector(self):
+        """The packed (ij|kl) vector at hfscf's canonical index (single-molecule contexts)."""
+        npair = self.nbf * (self.nbf + 1) // 2
+        v = np.zeros(npair * (npair + 1) // 2)
+        self._check(hfgpu().hfg_eri_vector(self._h, _ptr(v), len(v)))
+        return v
 
     def eri_block(self, s1, s2, s3, s4):
         sh = self.mol.shells()

@@ -348,6 +348,7 @@ void destroy_graphs(hfg_ctx* c) {
 
 int setup_geometry(hfg_ctx* c) {
     destroy_graphs(c);
+    c->eri_prog.clear();
     build_blocks(c);
     // block pairs (X >= Y) of every molecule
     c->bp_X.clear();
@@ -1718,6 +1719,68 @@ int hfg_eri_shell_quartet(hfg_ctx* c, int s1, int s2, int s3, int s4, double* bl
     return HFG_OK;
 }
 
+int hfg_eri_vector(hfg_ctx* c, double* out, long long count) {
+    int r = check_ready(c);
+    if (r) return r;
+    if (c->mols.size() != 1) return fail(c, HFG_ERR_ARG, "hfg_eri_vector: single-molecule contexts only");
+    const long long npair = (long long)c->nbf * (c->nbf + 1) / 2, n = npair * (npair + 1) / 2;
+    if (!out || count != n) return fail(c, HFG_ERR_ARG, "hfg_eri_vector: the vector must hold npair (npair + 1) / 2 values");
+    HFG_CHECK(c, hipSetDevice(c->device));
+    if (c->eri_prog.size() != c->classes.size()) {
+        // the cooperative program of every class (the J/K launches keep their own kernel choice)
+        std::vector<int4> words;
+        std::vector<std::pair<size_t, CoopHostProgram>> progs;
+        c->eri_lds.clear();
+        for (const ClassWork& cw : c->classes) {
+            CoopHostProgram hp = make_coop_program(cw.la, cw.lb, cw.lc, cw.ld, c->pure);
+            const size_t lds = hp.lds_bytes(false);
+            if (lds + kCoopStaticLds > (size_t)c->max_lds)
+                return fail(c, HFG_ERR_UNSUPPORTED, "hfg_eri_vector: a class's tables do not fit a workgroup's LDS");
+            c->eri_lds.push_back(lds);
+            const size_t base = words.size();
+            words.insert(words.end(), hp.words.begin(), hp.words.end());
+            progs.emplace_back(base, std::move(hp));
+        }
+        HFG_CHECK(c, c->d_eri_prog.upload(words, c->stream));
+        c->eri_prog.clear();
+        for (auto& pr : progs) c->eri_prog.push_back(coop_args_on_device(pr.second, c->d_eri_prog.p + pr.first));
+    }
+    HFG_CHECK(c, c->d_eri_out.ensure((size_t)n));
+    HFG_CHECK(c, hipMemsetAsync(c->d_eri_out.p, 0, (size_t)n * sizeof(double), c->stream));
+    JkArgs a{};
+    a.pairs = c->d_pairs.p;
+    a.psmean = c->d_psmean.p;
+    a.bp_off = c->d_bp_off.p;
+    a.bp_cnt = c->d_bp_cnt.p;
+    a.blk_nf = c->d_blk_nf.p;
+    a.pp0 = c->d_pp0.p;
+    a.pp1 = c->d_pp1.p;
+    a.pp2 = c->d_pp2.p;
+    a.boys = c->d_boys.p;
+    a.D = c->d_D.p;  // read by the kernel's density-block loads, unused by the export
+    a.mols = c->d_mols.p;
+    a.tol = c->tol;
+    a.psq = c->d_psq.p;
+    a.elem_tol = c->elem_tol;
+    a.prim_cut = c->opt.integral_tol / 1e4;
+    a.slab = c->d_slab.p;
+    a.dmax = c->d_dmax.p;
+    a.dtol = 0.0;
+    for (size_t k = 0; k < c->classes.size(); ++k) {
+        const ClassWork& cw = c->classes[k];
+        JkArgs b = a;
+        b.items = c->d_items.p + cw.dev_off;  // the per-shell items
+        b.queue = c->d_queue.p + k;
+        CoopArgs p = c->eri_prog[k];
+        p.eri_out = c->d_eri_out.p;
+        HFG_CHECK(c, launch_zero_ints(b.queue, 1, c->stream));
+        HFG_CHECK(c, launch_jk_coop(b, p, c->eri_lds[k], 0, (int)cw.items.size(), LaunchShape{0, 0}, c->stream));
+    }
+    HFG_CHECK(c, hipMemcpyAsync(out, c->d_eri_out.p, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HFG_CHECK(c, hipStreamSynchronize(c->stream));
+    return HFG_OK;
+}
+
 int hfg_boys_probe(hfg_ctx* c, int m, int n, const double* T, double* F, int closed_form) {
     if (!c) return HFG_ERR_ARG;
     if (!T || !F || n < 0 || m < 0 || m > kBoysMaxM || (closed_form && m > 1))
@@ -1836,6 +1899,8 @@ void hfg_destroy(hfg_ctx* c) {
     c->d_gcnt.release();
     c->d_gc_p0.release();
     c->d_one_sh.release();
+    c->d_eri_prog.release();
+    c->d_eri_out.release();
     c->d_one_pairs.release();
     c->d_one_alpha.release();
     c->d_one_coef.release();

@@ -318,6 +318,24 @@ __global__ __launch_bounds__(kCoopBlock) void jk_coop_kernel(JkArgs a, CoopArgs 
                 I = src;
             }
             __syncthreads();
+            if (p.eri_out) {
+                // export: every canonical element (i >= j, k >= l, ij >= kl; hfscf's index_ijkl,
+                // hfscf: src/math/hfscf_math.hpp:90) of the quartet, once. Each unordered pair of
+                // shell pairs is one quartet of the lists, so only a shell paired with itself (i < j
+                // within it) or a shell pair paired with itself (ij < kl) repeats an element.
+                const bool same_ab = bp.sa == bp.sb, same_cd = kp.sa == kp.sb, same_q = same && i == j;
+                auto tri = [](long long x, long long y) { return x >= y ? x * (x + 1) / 2 + y : y * (y + 1) / 2 + x; };
+                for (int o = lane; o < NA * NB * NC * ND; o += nl) {
+                    const int u = o % ND, r = (o / ND) % NC, qb = (o / (ND * NC)) % NB, pa = o / (ND * NC * NB);
+                    const int fi = bp.ao_a + pa, fj = bp.ao_b + qb, fk = kp.ao_a + r, fl = kp.ao_b + u;
+                    if ((same_ab && fi < fj) || (same_cd && fk < fl)) continue;
+                    const long long ij = tri(fi, fj), kl = tri(fk, fl);
+                    if (same_q && ij < kl) continue;
+                    p.eri_out[tri(ij, kl)] = (double)I[o];
+                }
+                __syncthreads();  // the next quartet reuses every LDS region
+                continue;
+            }
             // digestion: J tiles 2 s D I, K tiles s D I (J = T + T^T, K = T + T^T as in jk_kernel)
             const R sj = (R)(2.0 * s), sk = (R)s;
             const int la0 = bp.slot_a * NA, lb0 = bp.slot_b * NB, lc0 = kp.slot_a * NC, ld0 = kp.slot_b * ND;
@@ -360,6 +378,7 @@ __global__ __launch_bounds__(kCoopBlock) void jk_coop_kernel(JkArgs a, CoopArgs 
             __syncthreads();  // the next quartet reuses every LDS region
         }
         __syncthreads();
+        if (p.eri_out) continue;  // an export has no tiles (the item may have no slab)
         double* out = a.slab + w.slab;
         for (int k = lane; k < ntile; k += nl) out[k] = (double)tiles[k];
     }

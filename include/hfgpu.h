@@ -195,6 +195,16 @@ int hfg_oneint(hfg_ctx* ctx, double* S, double* T, double* V);
  * hfscf_oskineticderiv1.cpp:8, hfscf_osnuclearderiv1.cpp:15). */
 int hfg_oneint_gradient(hfg_ctx* ctx, const double* D, const double* Q, double* grad);
 
+/* The packed two-electron integral vector of the context's single molecule on the GPU: out[count]
+ * receives (ij|kl) at hfscf's canonical index index_ijkl = t(t(i,j), t(k,l)), t(x,y) =
+ * max(x,y)(max(x,y)+1)/2 + min(x,y), count = npair (npair + 1) / 2 with npair = nbf (nbf + 1) / 2,
+ * in the output basis, with the context's screening (quartets and elements hfscf's screen drops
+ * stay 0; its Cartesian diagonal), as hfscf stores them for post-SCF methods and the AO -> MO
+ * transform. Replaces: the stored repulsion vector of hf_solver (hfscf: src/hfscf/hfscf.cpp:76,
+ * src/basis/hfscf_trans_basis.cpp:7, src/math/hfscf_math.hpp:90; built by
+ * src/integrals/hfscf_oseri.cpp:22-246). */
+int hfg_eri_vector(hfg_ctx* ctx, double* out, long long count);
+
 /* Autotuner: times every class launch alone over candidate launch shapes (threads per
  * workgroup x persistent workgroups) on the density of the most recent build and keeps the
  * fastest per class. what: 1 = J/K, 2 = gradient, 3 = both. */

@@ -41,15 +41,16 @@ def _quartets(L):
 
 
 def _low_classes(L):
-    # one quartet of every class from L = 4 up to 6 (la >= lb, lc >= ld), on shells of different
-    # atoms where the basis has them: the classes the library may also run on these kernels
+    # one quartet of every class from L = 0 up to 6 (la >= lb, lc >= ld), on shells of different
+    # atoms where the basis has them: the classes the library may also run on these kernels (from
+    # L = 4 for J/K, every class for the ERI export)
     by_l = {l: [i for i in range(len(L)) if L[i] == l] for l in range(3)}
     qs = []
     for la in range(3):
         for lb in range(la + 1):
             for lc in range(3):
                 for ld in range(lc + 1):
-                    if not 4 <= la + lb + lc + ld <= 6:
+                    if not 0 <= la + lb + lc + ld <= 6:
                         continue
                     pick = [by_l[l][(k * 3 + 1) % len(by_l[l])] for k, l in enumerate((la, lb, lc, ld))]
                     qs.append(tuple(pick))
@@ -79,7 +80,7 @@ def test_coop_programs_against_oracle(evaluator, basis, pure):
         assert 0 < lds <= 64 * 1024 - 2048  # FP64 LDS of the launch fits a 64 KB workgroup
         worst = max(worst, float(np.abs(out.reshape(nf) - o.shell_quartet(*q)).max()))
     assert worst < 1e-14, worst
-    assert len(_low_classes(L)) == 22 - 3  # every class from L = 4 to 6
+    assert len(_low_classes(L)) == 36 - 3  # every class (la >= lb, lc >= ld) from L = 0 to 6
 
 
 def test_coop_gradient_programs_against_oracle(evaluator):
