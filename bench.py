@@ -280,6 +280,38 @@ def oneint_figures(mol, D, F):
     return out
 
 
+def eri_export_figures():
+    """The packed ERI vector (hfscf's canonical index, screened as hfscf stores it) of (H2O)8 /
+    cc-pVDZ from the GPU (hfg_eri_vector, the device-to-host copy included). The host-side time of
+    the same vector is in cpu_baseline["eri_export"]."""
+    mol = H.Molecule.from_xyz(os.path.join(ROOT, "tests", "golden", "h2o_8.xyz"), "cc-pvdz")
+    ctx = H.GpuContext(mol)
+    ctx.eri_vector()  # programs, buffers
+    t = time.perf_counter()
+    v = ctx.eri_vector()
+    t_gpu = time.perf_counter() - t
+    ctx.close()
+    out = {"eri_export_workload": "(H2O)8 / cc-pVDZ, 192 functions", "eri_export_values": int(v.size),
+           "eri_export_nonzero": int(np.count_nonzero(v)), "eri_export_gpu_seconds": t_gpu,
+           "eri_export_gpu_values_per_s": v.size / t_gpu}
+    log(f"ERI export (H2O)8: {v.size} values in {t_gpu:.3f} s on the GPU")
+    return out
+
+
+def cpu_eri_export():
+    """The oracle's stored-ERI build of the (H2O)8 / cc-pVDZ vector on the host cores (the CPU
+    side of the ERI export figure)."""
+    import oracle
+    mol = H.Molecule.from_xyz(os.path.join(ROOT, "tests", "golden", "h2o_8.xyz"), "cc-pvdz")
+    o = oracle.Oracle(mol.shells(), mol.geometry()[1], mol.pure, 1e-14, screen=True)
+    t = time.perf_counter()
+    o.build_eri()
+    sec = time.perf_counter() - t
+    threads = int(os.environ.get("OMP_NUM_THREADS") or 0) or min(os.cpu_count() or 1, 16)
+    log(f"ERI export (H2O)8: oracle {sec:.1f} s on {threads} threads")
+    return {"workload": "(H2O)8 / cc-pVDZ packed ERI vector", "seconds": sec, "threads": threads, "kind": "port"}
+
+
 def batch_figures(count, sample, seed=11):
     """Batching mode on the seeded (H2O)n clusters (n in 1..10, 6-31G*): all molecules in one
     lockstep SCF with one batched build per iteration (HIP graphs, autotuned launch shapes), then
@@ -427,11 +459,14 @@ def main():
                 ex.update(scf_figures(mol, scf, g64))
         if not args.no_scf:
             ex.update(oneint_figures(mol, scf["D"], scf["F"]))
+        ex.update(eri_export_figures())
         if args.batch_molecules > 0:
             ex.update(batch_figures(args.batch_molecules, args.solo_sample))
         out["extra"] = ex
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(mol)
+        if not args.no_extra:
+            out["cpu_baseline"]["eri_export"] = cpu_eri_export()
     print(json.dumps(out), flush=True)
 
 

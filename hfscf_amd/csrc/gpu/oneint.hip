@@ -7,11 +7,12 @@
 // Coulomb integrals R_tuv (Boys values from the device table), derivatives by the shifts
 // d/dA_x |a> = 2 alpha |a + 1_x> - a_x |a - 1_x>.
 //
-// One lane per shell pair (angular momenta up to 2, so the derivative shifts reach 3 on the bra
-// and, for T, 5 on the ket); the lane loops over the pair's primitive pairs and, for V, over the
-// nuclei. This is O(N^2 natom) work next to the O(N^4) two-electron build, so the kernels keep
-// their tables in per-lane arrays sized for the largest shells rather than specialising per
-// class.
+// One lane per shell pair for S and T, per shell pair and nucleus for V (added into V with
+// atomics), and per ordered shell pair and part (the overlap and kinetic terms, or one nucleus)
+// for the gradient terms; angular momenta up to 2, so the derivative shifts reach 3 on the bra and,
+// for T, 5 on the ket. This is O(N^2 natom) work next to the O(N^4) two-electron build, so the
+// kernels keep their tables in per-lane arrays sized for the largest shells rather than
+// specialising per class.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -215,17 +216,28 @@ __device__ inline void scatter(const OneArgs& g, int sa, int sb, const double* b
 
 constexpr int kBlk = 36;  // 6 x 6 Cartesian components of two d shells
 
-// S, T and V of shell pair k (sa >= sb) of the pair list.
-__global__ __launch_bounds__(64) void oneint_kernel(OneArgs g) {
+// Adds block blk (na x nb) of shell pair (sa, sb) into M with atomics: at (a, b) and, for two
+// different shells, at (b, a) (the block of a shell with itself already covers both).
+__device__ inline void scatter_add(const OneArgs& g, int sa, int sb, const double* blk, double* M) {
+    const OneShell A = g.sh[sa], B = g.sh[sb];
+    const int na = g.pure ? 2 * A.L + 1 : ncart1(A.L), nb = g.pure ? 2 * B.L + 1 : ncart1(B.L);
+    for (int p = 0; p < na; ++p)
+        for (int q = 0; q < nb; ++q) {
+            atomicAdd(&M[(size_t)(A.ao + p) * g.n + B.ao + q], blk[p * nb + q]);
+            if (sa != sb) atomicAdd(&M[(size_t)(B.ao + q) * g.n + A.ao + p], blk[p * nb + q]);
+        }
+}
+
+// S and T of shell pair k (sa >= sb) of the pair list.
+__global__ __launch_bounds__(64) void overlap_kinetic_kernel(OneArgs g) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= g.npairs) return;
     const int sa = g.pairs[k].x, sb = g.pairs[k].y;
     const OneShell A = g.sh[sa], B = g.sh[sb];
     const int nca = ncart1(A.L), ncb = ncart1(B.L);
-    double Sb[kBlk], Tb[kBlk], Vb[kBlk];
-    for (int x = 0; x < kBlk; ++x) Sb[x] = Tb[x] = Vb[x] = 0.0;
+    double Sb[kBlk], Tb[kBlk];
+    for (int x = 0; x < kBlk; ++x) Sb[x] = Tb[x] = 0.0;
     Herm1<kOneMaxL + 1, kOneMaxL + 3> E[3];
-    double R[kRSize];
     for (int i = 0; i < A.np; ++i)
         for (int j = 0; j < B.np; ++j) {
             const PrimPairDev pp = prim_pair_dev(g, sa, i, sb, j);
@@ -243,40 +255,64 @@ __global__ __launch_bounds__(64) void oneint_kernel(OneArgs g) {
                     Sb[x * ncb + y] += pp.K * sx * sy * sz;
                     Tb[x * ncb + y] += pp.K * (tx * sy * sz + sx * ty * sz + sx * sy * tz);
                 }
-            const double pre = 2.0 * M_PI / pp.p * pp.K;
-            for (int c = 0; c < g.natom; ++c) {
-                const double pc[3] = {pp.P[0] - g.xyz[3 * c], pp.P[1] - g.xyz[3 * c + 1], pp.P[2] - g.xyz[3 * c + 2]};
-                hermite_r(A.L + B.L, pp.p, pc, g.boys, R);
-                for (int x = 0; x < nca; ++x)
-                    for (int y = 0; y < ncb; ++y) {
-                        int a[3], b[3];
-                        cart_lmn(A.L, x, a);
-                        cart_lmn(B.L, y, b);
-                        double acc = 0.0;
-                        for (int t = 0; t <= a[0] + b[0]; ++t)
-                            for (int u = 0; u <= a[1] + b[1]; ++u)
-                                for (int v = 0; v <= a[2] + b[2]; ++v)
-                                    acc += E[0].get(a[0], b[0], t) * E[1].get(a[1], b[1], u) * E[2].get(a[2], b[2], v) *
-                                           R[ridx(t, u, v)];
-                        Vb[x * ncb + y] -= g.Z[c] * pre * acc;
-                    }
-            }
         }
     double o[kBlk];
     block_out(A.L, B.L, g.pure, Sb, o);
     scatter(g, sa, sb, o, g.S);
     block_out(A.L, B.L, g.pure, Tb, o);
     scatter(g, sa, sb, o, g.T);
+}
+
+// The attraction of nucleus c in V for shell pair k (one lane per pair and nucleus, added into
+// the zeroed V with atomics).
+__global__ __launch_bounds__(64) void nuclear_kernel(OneArgs g) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)g.npairs * g.natom) return;
+    const int k = (int)(t / g.natom), c = (int)(t - (long long)k * g.natom);
+    const int sa = g.pairs[k].x, sb = g.pairs[k].y;
+    const OneShell A = g.sh[sa], B = g.sh[sb];
+    const int nca = ncart1(A.L), ncb = ncart1(B.L);
+    double Vb[kBlk];
+    for (int x = 0; x < kBlk; ++x) Vb[x] = 0.0;
+    Herm1<kOneMaxL + 1, kOneMaxL + 1> E[3];
+    double R[kRSize];
+    const double zc = g.Z[c];
+    for (int i = 0; i < A.np; ++i)
+        for (int j = 0; j < B.np; ++j) {
+            const PrimPairDev pp = prim_pair_dev(g, sa, i, sb, j);
+            for (int d = 0; d < 3; ++d) E[d].build(A.L, B.L, pp.p, pp.PA[d], pp.PB[d]);
+            const double pre = 2.0 * M_PI / pp.p * pp.K;
+            const double pc[3] = {pp.P[0] - g.xyz[3 * c], pp.P[1] - g.xyz[3 * c + 1], pp.P[2] - g.xyz[3 * c + 2]};
+            hermite_r(A.L + B.L, pp.p, pc, g.boys, R);
+            for (int x = 0; x < nca; ++x)
+                for (int y = 0; y < ncb; ++y) {
+                    int a[3], b[3];
+                    cart_lmn(A.L, x, a);
+                    cart_lmn(B.L, y, b);
+                    double acc = 0.0;
+                    for (int tt = 0; tt <= a[0] + b[0]; ++tt)
+                        for (int u = 0; u <= a[1] + b[1]; ++u)
+                            for (int v = 0; v <= a[2] + b[2]; ++v)
+                                acc += E[0].get(a[0], b[0], tt) * E[1].get(a[1], b[1], u) * E[2].get(a[2], b[2], v) *
+                                       R[ridx(tt, u, v)];
+                    Vb[x * ncb + y] -= zc * pre * acc;
+                }
+        }
+    double o[kBlk];
     block_out(A.L, B.L, g.pure, Vb, o);
-    scatter(g, sa, sb, o, g.V);
+    scatter_add(g, sa, sb, o, g.V);
 }
 
 // One-electron gradient terms of ordered shell pair (sa, sb) (every pair, both orders, as the
 // sum over i, j of the gradient formula runs), added into grad with atomics:
 //   2 D_ab d(T + V)_ab/dX - 2 Q_ab dS_ab/dX.
 __global__ __launch_bounds__(64) void oneint_grad_kernel(OneArgs g, const double* D, const double* Q, double* grad) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= g.nshell * g.nshell) return;
+    // one lane per ordered shell pair and part: part natom = overlap and kinetic terms, part c <
+    // natom = the attraction of nucleus c
+    const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int parts = g.natom + 1;
+    if (tid >= (long long)g.nshell * g.nshell * parts) return;
+    const int k = (int)(tid / parts), part = (int)(tid - (long long)k * parts);
     const int sa = k / g.nshell, sb = k - sa * g.nshell;
     const OneShell A = g.sh[sa], B = g.sh[sb];
     const int nca = ncart1(A.L), ncb = ncart1(B.L);
@@ -292,6 +328,7 @@ __global__ __launch_bounds__(64) void oneint_grad_kernel(OneArgs g, const double
             for (int d = 0; d < 3; ++d) E[d].build(A.L + 1, B.L + 3, pp.p, pp.PA[d], pp.PB[d]);
             const double sq = sqrt(M_PI / pp.p);
             // overlap and kinetic: d/dA = 2a |a+1> - a_x |a-1>; d/dB = -d/dA
+            if (part == g.natom)
             for (int x = 0; x < nca; ++x)
                 for (int y = 0; y < ncb; ++y) {
                     const double wd = WD[x * ncb + y], wq = WQ[x * ncb + y];
@@ -319,7 +356,8 @@ __global__ __launch_bounds__(64) void oneint_grad_kernel(OneArgs g, const double
             // nuclear attraction: bra and ket shifts, and the operator's centre by translational
             // invariance (dC = -(dA + dB))
             const double pre = 2.0 * M_PI / pp.p * pp.K;
-            for (int c = 0; c < g.natom; ++c) {
+            if (part < g.natom) {
+                const int c = part;
                 const double pc[3] = {pp.P[0] - g.xyz[3 * c], pp.P[1] - g.xyz[3 * c + 1], pp.P[2] - g.xyz[3 * c + 2]};
                 hermite_r(A.L + B.L + 1, pp.p, pc, g.boys, R);
                 auto vint = [&](const int* a, const int* b) {
@@ -368,14 +406,18 @@ __global__ __launch_bounds__(64) void oneint_grad_kernel(OneArgs g, const double
 
 hipError_t launch_oneint(const OneArgs& g, hipStream_t s) {
     if (g.npairs == 0) return hipSuccess;
-    hipLaunchKernelGGL(oneint_kernel, dim3((g.npairs + 63) / 64), dim3(64), 0, s, g);
+    hipError_t e = hipMemsetAsync(g.V, 0, (size_t)g.n * g.n * sizeof(double), s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(overlap_kinetic_kernel, dim3((g.npairs + 63) / 64), dim3(64), 0, s, g);
+    const long long nv = (long long)g.npairs * g.natom;
+    hipLaunchKernelGGL(nuclear_kernel, dim3((unsigned)((nv + 63) / 64)), dim3(64), 0, s, g);
     return hipGetLastError();
 }
 
 hipError_t launch_oneint_grad(const OneArgs& g, const double* D, const double* Q, double* grad, hipStream_t s) {
-    const int n = g.nshell * g.nshell;
+    const long long n = (long long)g.nshell * g.nshell * (g.natom + 1);
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(oneint_grad_kernel, dim3((n + 63) / 64), dim3(64), 0, s, g, D, Q, grad);
+    hipLaunchKernelGGL(oneint_grad_kernel, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, s, g, D, Q, grad);
     return hipGetLastError();
 }
 
