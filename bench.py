@@ -158,16 +158,20 @@ def load_traffic(kernel, workload):
 
 
 def dominant_roofline(classes, workload):
-    """Roofline of the dominant J/K kernel (the class launch with the longest duration in the last
-    build): model FLOPs of its surviving quartets / its HIP-event duration, against FP64 vector peak."""
-    c = max(classes, key=lambda c: c["kernel_ms"])
+    """Roofline of the dominant J/K kernel, the class launch with the most model FLOPs of the last
+    build: those FLOPs / its HIP-event duration in that build (the launches of one build run
+    concurrently, so a small class that waits for compute units can last longer than the kernel
+    that does the most work), against the FP64 vector peak."""
+    c = max(classes, key=lambda c: c["model_flops"])
     cls = "".join(str(x) for x in c["cls"])
     ach = c["model_flops"] / (c["kernel_ms"] * 1e-3) / 1e12
     traffic, entry = load_traffic(f"{c['kernel']}_{cls}_fp64", workload)
     out = {"bound": "valu", "kernel": f"{c['kernel']} ({cls[:2]}|{cls[2:]}) fp64", "achieved": ach, "peak": PEAK_FP64,
            "unit": "TFLOP/s", "frac": ach / PEAK_FP64, "traffic": traffic,
            "model_gflop_per_launch": c["model_flops"] / 1e9, "kernel_ms": c["kernel_ms"],
-           "quartets_per_launch": c["quartets"]}
+           "quartets_per_launch": c["quartets"],
+           "selection": "class launch with the most model FLOPs in the build",
+           "longest_launch": "".join(str(x) for x in max(classes, key=lambda c: c["kernel_ms"])["cls"])}
     if entry:
         out["traffic_source"] = entry.get("source")
         out["arithmetic_intensity_flop_per_byte"] = c["model_flops"] / traffic if traffic else None
