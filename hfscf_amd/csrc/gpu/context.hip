@@ -63,17 +63,27 @@ void assign_streams(hfg_ctx* c, bool grad, const std::vector<double>& w) {
     });
     std::vector<int>& order = grad ? c->grad_order : c->class_order;
     order.assign(ord.begin(), ord.end());
-    double load[kSideStreams] = {0};
+    // the balance is over hardware queues: side streams on one queue count as one, the first of
+    // them takes the queue's classes, and a queue shared with the shared stream starts with the
+    // classes that run there
     const int nside = std::max(1, std::min(c->nside, kSideStreams));
+    double load[kSideStreams] = {0};
+    bool rep[kSideStreams] = {};
+    for (int s = 0; s < nside; ++s) rep[s] = c->side_queue[s] == s;
+    for (size_t k = 0; k < c->classes.size(); ++k) {
+        const ClassWork& cw = c->classes[k];
+        const bool heavy = class_spills(cw.la, cw.lb, cw.lc, cw.ld, grad) && !(grad ? cw.gcoop : cw.coop);
+        if (heavy && c->heavy_queue >= 0) load[c->heavy_queue] += w[k];
+    }
     for (size_t k : ord) {
         const ClassWork& cw = c->classes[k];
         if (class_spills(cw.la, cw.lb, cw.lc, cw.ld, grad) && !(grad ? cw.gcoop : cw.coop)) {
             map[k] = -1;
             continue;
         }
-        int best = 0;
-        for (int s = 1; s < nside; ++s)
-            if (load[s] < load[best]) best = s;
+        int best = -1;
+        for (int s = 0; s < nside; ++s)
+            if (rep[s] && (best < 0 || load[s] < load[best])) best = s;
         map[k] = best;
         load[best] += w[k];
     }
@@ -1370,6 +1380,35 @@ int hfg_create(hfg_ctx** out, int device, const hfg_options* opt) {
             hfg_destroy(c);
             return HFG_ERR_HIP;
         }
+    {
+        // which hardware queue serves each side stream (HIP deals streams to a few queues per
+        // process, and a queue runs its kernels in order)
+        DevBuf<unsigned long long> q;
+        std::vector<unsigned long long> addr(c->nside + 1, 0);
+        bool ok = q.ensure(c->nside + 1) == hipSuccess;
+        for (int k = 0; ok && k < c->nside; ++k) ok = launch_queue_probe(q.p, k, c->side[k]) == hipSuccess;
+        ok = ok && launch_queue_probe(q.p, c->nside, c->heavy) == hipSuccess;
+        for (int k = 0; ok && k < c->nside; ++k) ok = hipStreamSynchronize(c->side[k]) == hipSuccess;
+        ok = ok && hipStreamSynchronize(c->heavy) == hipSuccess &&
+             hipMemcpy(addr.data(), q.p, addr.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost) == hipSuccess;
+        q.release();
+        if (!ok) {
+            hfg_destroy(c);
+            return HFG_ERR_HIP;
+        }
+        for (int k = 0; k <= c->nside; ++k) {
+            int id = k;
+            for (int j = 0; j < k; ++j)
+                if (addr[j] == addr[k]) {
+                    id = (j < c->nside) ? c->side_queue[j] : j;
+                    break;
+                }
+            if (k < c->nside)
+                c->side_queue[k] = id;
+            else
+                c->heavy_queue = id < c->nside ? id : -1;
+        }
+    }
     std::vector<double> tab = boys_table();
     if (c->d_boys.upload(tab, c->stream) != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess) {
         hfg_destroy(c);

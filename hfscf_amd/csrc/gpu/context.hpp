@@ -144,6 +144,10 @@ struct hfg_ctx {
     std::vector<int> class_stream;  // side stream of each class's J/K launch (-1: the shared stream)
     std::vector<int> grad_stream;   // the same for the gradient launches
     std::vector<int> class_order, grad_order;  // launch order of the classes (J/K, gradient)
+    // hardware queue of each side stream and of the shared stream (equal values: one queue, whose
+    // kernels run one after another); the stream balance spreads the classes over the queues
+    int side_queue[hfg::kSideStreams] = {};
+    int heavy_queue = -1;
     hfg_options opt{};
     std::string err;
     bool have_basis = false, have_geom = false;

@@ -25,4 +25,7 @@ hipError_t launch_gather_chunks(const GatherArgs& a, const GatherChunk* chunks, 
 hipError_t launch_boys_probe(int m, const double* T, const double* tab, double* out, int n, int closed_form,
                              hipStream_t s);
 
+// Writes the address of the hardware queue that runs stream s's kernels into out[k] (device).
+hipError_t launch_queue_probe(unsigned long long* out, int k, hipStream_t s);
+
 }  // namespace hfg

@@ -122,6 +122,17 @@ hipError_t launch_zero_ints(int* p, int n, hipStream_t s) {
     return hipGetLastError();
 }
 
+// The address of the hardware queue (HSA queue descriptor) the dispatch came from, into out[k]:
+// streams that HIP placed on one hardware queue run their kernels one after another.
+__global__ void queue_probe_kernel(unsigned long long* out, int k) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) out[k] = (unsigned long long)(size_t)__builtin_amdgcn_queue_ptr();
+}
+
+hipError_t launch_queue_probe(unsigned long long* out, int k, hipStream_t s) {
+    hipLaunchKernelGGL(queue_probe_kernel, dim3(1), dim3(64), 0, s, out, k);
+    return hipGetLastError();
+}
+
 // Plain streaming copy, 16 bytes per lane and iteration (a kernel of known traffic, to scale a
 // profiler's HBM byte counters).
 __global__ __launch_bounds__(256) void copy_kernel(const double2* __restrict__ src, double2* __restrict__ dst,
