@@ -303,22 +303,28 @@ __global__ __launch_bounds__(64) void nuclear_kernel(OneArgs g) {
     scatter_add(g, sa, sb, o, g.V);
 }
 
-// One-electron gradient terms of ordered shell pair (sa, sb) (every pair, both orders, as the
-// sum over i, j of the gradient formula runs), added into grad with atomics:
+// One-electron gradient terms of shell pair (sa, sb) and its mirror (sb, sa) (the sum over i, j of
+// the gradient formula runs over both orders), added into grad with atomics:
 //   2 D_ab d(T + V)_ab/dX - 2 Q_ab dS_ab/dX.
 __global__ __launch_bounds__(64) void oneint_grad_kernel(OneArgs g, const double* D, const double* Q, double* grad) {
-    // one lane per ordered shell pair and part: part natom = overlap and kinetic terms, part c <
-    // natom = the attraction of nucleus c
+    // one lane per shell pair (sa >= sb) and part: part natom = overlap and kinetic terms, part
+    // c < natom = the attraction of nucleus c. D and Q are symmetric, so the ordered pair (sb, sa)
+    // adds what (sa, sb) adds: a pair of two shells counts twice.
     const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const int parts = g.natom + 1;
-    if (tid >= (long long)g.nshell * g.nshell * parts) return;
+    if (tid >= (long long)g.npairs * parts) return;
     const int k = (int)(tid / parts), part = (int)(tid - (long long)k * parts);
-    const int sa = k / g.nshell, sb = k - sa * g.nshell;
+    const int sa = g.pairs[k].x, sb = g.pairs[k].y;
     const OneShell A = g.sh[sa], B = g.sh[sb];
     const int nca = ncart1(A.L), ncb = ncart1(B.L);
     double WD[kBlk], WQ[kBlk];
     weight_cart(A.L, B.L, g.pure, D, g.n, A.ao, B.ao, WD);
     weight_cart(A.L, B.L, g.pure, Q, g.n, A.ao, B.ao, WQ);
+    if (sa != sb)
+        for (int x = 0; x < nca * ncb; ++x) {
+            WD[x] *= 2.0;
+            WQ[x] *= 2.0;
+        }
     double gA[3] = {0.0, 0.0, 0.0}, gB[3] = {0.0, 0.0, 0.0};
     Herm1<kOneMaxL + 2, kOneMaxL + 4> E[3];
     double R[kRSize];
@@ -415,7 +421,7 @@ hipError_t launch_oneint(const OneArgs& g, hipStream_t s) {
 }
 
 hipError_t launch_oneint_grad(const OneArgs& g, const double* D, const double* Q, double* grad, hipStream_t s) {
-    const long long n = (long long)g.nshell * g.nshell * (g.natom + 1);
+    const long long n = (long long)g.npairs * (g.natom + 1);
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(oneint_grad_kernel, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, s, g, D, Q, grad);
     return hipGetLastError();
@@ -491,6 +497,12 @@ int hfg_oneint_gradient(hfg_ctx* c, const double* D, const double* Q, double* gr
     if (!D || !Q || !grad) return one_fail(c, HFG_ERR_ARG, "hfg_oneint_gradient: null argument");
     hfg::OneArgs g;
     if (int r = one_upload(c, g)) return r;
+    std::vector<int2> pairs;
+    for (int a = 0; a < c->nshell; ++a)
+        for (int b = 0; b <= a; ++b) pairs.push_back(make_int2(a, b));
+    HFG_CHECK(c, c->d_one_pairs.upload(pairs, c->stream));
+    g.pairs = c->d_one_pairs.p;
+    g.npairs = (int)pairs.size();
     const size_t nn = (size_t)c->nbf * c->nbf;
     const int nout = 3 * c->natom;
     HFG_CHECK(c, c->d_one_buf.ensure(2 * nn + nout));

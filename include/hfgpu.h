@@ -190,7 +190,7 @@ int hfg_batch_layout(const hfg_ctx* ctx, int* nmol, int* nbf, long long* moff, i
  * src/integrals/hfscf_osoverlap.cpp:10, hfscf_osnuclear.cpp:17-83). */
 int hfg_oneint(hfg_ctx* ctx, double* S, double* T, double* V);
 /* grad (natom x 3, host, added into) += 2 sum_ij D_ij d(T+V)_ij/dX - 2 sum_ij Q_ij dS_ij/dX on the
- * GPU, D the half density and Q = D F D. Replaces: the dTdX, dVndX and dSdX terms of
+ * GPU, D the half density and Q = D F D (both symmetric, as in RHF). Replaces: the dTdX, dVndX and dSdX terms of
  * calc_scf_gradient_rhf (hfscf: src/gradient/hfscf_grad_rhf.cpp:143-167; hfscf_osoverlapderiv1.cpp:8,
  * hfscf_oskineticderiv1.cpp:8, hfscf_osnuclearderiv1.cpp:15). */
 int hfg_oneint_gradient(hfg_ctx* ctx, const double* D, const double* Q, double* grad);
