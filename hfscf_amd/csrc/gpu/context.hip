@@ -46,7 +46,7 @@ hipStream_t shared_scratch_stream(int device) {
 }
 
 // Streams and launch order of the class launches (J/K, or the gradient when grad): the classes
-// that spill run on the device's shared stream (-1); the others are balanced over the side
+// of the shared stream (on_shared_stream) run there (-1); the others are balanced over the side
 // streams, onto the least loaded one, by the weights w (the FLOP model at set-up, the measured
 // kernel times after autotuning). The launch order is the cooperative-kernel classes first (one
 // quartet per workgroup at a time: few workgroups on long serial chains, whose latency then
@@ -72,12 +72,11 @@ void assign_streams(hfg_ctx* c, bool grad, const std::vector<double>& w) {
     for (int s = 0; s < nside; ++s) rep[s] = c->side_queue[s] == s;
     for (size_t k = 0; k < c->classes.size(); ++k) {
         const ClassWork& cw = c->classes[k];
-        const bool heavy = class_spills(cw.la, cw.lb, cw.lc, cw.ld, grad) && !(grad ? cw.gcoop : cw.coop);
-        if (heavy && c->heavy_queue >= 0) load[c->heavy_queue] += w[k];
+        if (on_shared_stream(c, cw, grad) && c->heavy_queue >= 0) load[c->heavy_queue] += w[k];
     }
     for (size_t k : ord) {
         const ClassWork& cw = c->classes[k];
-        if (class_spills(cw.la, cw.lb, cw.lc, cw.ld, grad) && !(grad ? cw.gcoop : cw.coop)) {
+        if (on_shared_stream(c, cw, grad)) {
             map[k] = -1;
             continue;
         }

@@ -14,6 +14,7 @@
 #include "kernels.hpp"
 #include "launch_bounds.hpp"
 #include "oneint.hpp"
+#include "private_segments.hpp"
 
 namespace hfg {
 
@@ -311,13 +312,30 @@ inline hipError_t for_each_class_concurrent(hfg_ctx* c, hipStream_t main, F&& fn
     return join_shared ? hipStreamWaitEvent(main, c->ev_heavy_join, 0) : hipSuccess;
 }
 
-// Stream of a class launch (J/K or gradient): a register kernel that spills always runs on the
-// device's shared stream, also while autotuning tries it for a class that is on a side stream.
+// Largest scratch per lane of a gradient register kernel that runs concurrently with others on
+// the side streams: the J/K register kernels of (dp|dp) and (dd|pp), 4,088 bytes per lane, run
+// there in every build.
+constexpr int kGradConcurrentBytes = 4096;
+
+// Whether a class launch runs on the device's shared stream (one kernel at a time): J/K register
+// kernels whose scratch bound is above kJkSharedBytes; gradient register kernels whose private
+// segment in this build (the larger of the FP64 and mixed kernels) is above kGradConcurrentBytes
+// (the scratch bound decides where the table has no entry).
+inline bool on_shared_stream(const hfg_ctx* c, const ClassWork& cw, bool grad) {
+    if (grad ? cw.gcoop : cw.coop) return false;
+    if (!grad) return class_spills(cw.la, cw.lb, cw.lc, cw.ld, false);
+    const int b64 = private_segment_bytes(true, false, cw.la, cw.lb, cw.lc, cw.ld, c->pure);
+    const int b32 = private_segment_bytes(true, true, cw.la, cw.lb, cw.lc, cw.ld, c->pure);
+    if (b64 < 0 || b32 < 0) return class_spills(cw.la, cw.lb, cw.lc, cw.ld, true);
+    return std::max(b64, b32) > kGradConcurrentBytes;
+}
+
+// Stream of a class launch (J/K or gradient): a kernel of the shared stream always runs there,
+// also while autotuning tries it for a class that is on a side stream.
 inline hipStream_t class_launch_stream(hfg_ctx* c, size_t k, bool grad) {
     const ClassWork& cw = c->classes[k];
     const int s = (grad ? c->grad_stream : c->class_stream)[k];
-    const bool coop = grad ? cw.gcoop : cw.coop;
-    if (s < 0 || (!coop && class_spills(cw.la, cw.lb, cw.lc, cw.ld, grad))) return c->heavy;
+    if (s < 0 || on_shared_stream(c, cw, grad)) return c->heavy;
     return c->side[s];
 }
 
