@@ -22,8 +22,8 @@ namespace hfg {
 // Three wavefronts per SIMD: the FP64 kernel then holds 168 VGPRs and keeps 84 bytes per lane in
 // scratch, against 185 VGPRs and two wavefronts without the attribute; alone on the (H2O)16 /
 // cc-pVDZ build it took 5.0 ms instead of 5.9 (the FP32 kernel fits four wavefronts either way).
-// The (ps|ss) group kernel, forced the same way, spilled 340 bytes per lane and slowed from 8.1 to
-// 11.4 ms.
+// The FP64 (ps|ss) group kernel, forced the same way, spilled 340 bytes per lane and slowed from
+// 8.1 to 11.4 ms.
 template <typename R>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void jk_gc_ssss_kernel(JkArgs a, GcArgs g) {
     __shared__ double tiles[kRegions * kBlockFuncs * kBlockFuncs];

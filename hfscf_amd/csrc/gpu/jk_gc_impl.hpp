@@ -20,8 +20,13 @@
 
 namespace hfg {
 
+// The FP32 kernel of the mixed builds at four wavefronts per SIMD (128 VGPRs and 60 bytes of
+// scratch per lane for (ps|ss), against 162 VGPRs and three wavefronts unconstrained); the FP64
+// kernel unconstrained (forced to three wavefronts it kept 340 bytes per lane in scratch and
+// slowed from 8.1 to 11.4 ms on the (H2O)16 build).
 template <int LA, int LB, int LC, int LD, bool PURE, typename R>
-__global__ __launch_bounds__(256) void jk_gc_kernel(JkArgs a, GcArgs g) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(std::is_same_v<R, float> ? 4 : 1))) void
+jk_gc_kernel(JkArgs a, GcArgs g) {
     constexpr int LT = LA + LB + LC + LD;
     static_assert(LT > 0, "(ss|ss) has its own group kernel");
     constexpr int NA = PURE && LA > 0 ? 2 * LA + 1 : ncart(LA);
