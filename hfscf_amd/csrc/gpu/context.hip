@@ -1816,6 +1816,7 @@ int hfg_eri_vector(hfg_ctx* c, double* out, long long count) {
     }
     HFG_CHECK(c, hipMemcpyAsync(out, c->d_eri_out.p, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HFG_CHECK(c, hipStreamSynchronize(c->stream));
+    c->d_eri_out.release();  // the vector grows as N^4; the context keeps no copy
     return HFG_OK;
 }
 
