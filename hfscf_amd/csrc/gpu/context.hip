@@ -357,7 +357,6 @@ void destroy_graphs(hfg_ctx* c) {
 
 int setup_geometry(hfg_ctx* c) {
     destroy_graphs(c);
-    c->eri_prog.clear();
     build_blocks(c);
     // block pairs (X >= Y) of every molecule
     c->bp_X.clear();
@@ -1764,25 +1763,7 @@ int hfg_eri_vector(hfg_ctx* c, double* out, long long count) {
     const long long npair = (long long)c->nbf * (c->nbf + 1) / 2, n = npair * (npair + 1) / 2;
     if (!out || count != n) return fail(c, HFG_ERR_ARG, "hfg_eri_vector: the vector must hold npair (npair + 1) / 2 values");
     HFG_CHECK(c, hipSetDevice(c->device));
-    if (c->eri_prog.size() != c->classes.size()) {
-        // the cooperative program of every class (the J/K launches keep their own kernel choice)
-        std::vector<int4> words;
-        std::vector<std::pair<size_t, CoopHostProgram>> progs;
-        c->eri_lds.clear();
-        for (const ClassWork& cw : c->classes) {
-            CoopHostProgram hp = make_coop_program(cw.la, cw.lb, cw.lc, cw.ld, c->pure);
-            const size_t lds = hp.lds_bytes(false);
-            if (lds + kCoopStaticLds > (size_t)c->max_lds)
-                return fail(c, HFG_ERR_UNSUPPORTED, "hfg_eri_vector: a class's tables do not fit a workgroup's LDS");
-            c->eri_lds.push_back(lds);
-            const size_t base = words.size();
-            words.insert(words.end(), hp.words.begin(), hp.words.end());
-            progs.emplace_back(base, std::move(hp));
-        }
-        HFG_CHECK(c, c->d_eri_prog.upload(words, c->stream));
-        c->eri_prog.clear();
-        for (auto& pr : progs) c->eri_prog.push_back(coop_args_on_device(pr.second, c->d_eri_prog.p + pr.first));
-    }
+    if (int r = check_private_segments(c, false)) return r;
     HFG_CHECK(c, c->d_eri_out.ensure((size_t)n));
     HFG_CHECK(c, hipMemsetAsync(c->d_eri_out.p, 0, (size_t)n * sizeof(double), c->stream));
     JkArgs a{};
@@ -1804,15 +1785,20 @@ int hfg_eri_vector(hfg_ctx* c, double* out, long long count) {
     a.slab = c->d_slab.p;
     a.dmax = c->d_dmax.p;
     a.dtol = 0.0;
+    a.eri_out = c->d_eri_out.p;
     for (size_t k = 0; k < c->classes.size(); ++k) {
+        // the FP64 per-lane kernel of the class, or its cooperative kernel where the J/K builds run
+        // that one (the per-shell items: a class on the shell-group lists has them too)
         const ClassWork& cw = c->classes[k];
         JkArgs b = a;
-        b.items = c->d_items.p + cw.dev_off;  // the per-shell items
+        b.items = c->d_items.p + cw.dev_off;
         b.queue = c->d_queue.p + k;
-        CoopArgs p = c->eri_prog[k];
-        p.eri_out = c->d_eri_out.p;
+        const int n_items = (int)cw.items.size();
         HFG_CHECK(c, launch_zero_ints(b.queue, 1, c->stream));
-        HFG_CHECK(c, launch_jk_coop(b, p, c->eri_lds[k], 0, (int)cw.items.size(), LaunchShape{0, 0}, c->stream));
+        if (cw.coop)
+            HFG_CHECK(c, launch_jk_coop(b, cw.coop_args, cw.coop_lds[0], 0, n_items, LaunchShape{0, 0}, c->stream));
+        else
+            HFG_CHECK(c, launch_jk(cw.la, cw.lb, cw.lc, cw.ld, c->pure, 0, b, n_items, LaunchShape{0, 0}, c->stream));
     }
     HFG_CHECK(c, hipMemcpyAsync(out, c->d_eri_out.p, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HFG_CHECK(c, hipStreamSynchronize(c->stream));
@@ -1938,7 +1924,6 @@ void hfg_destroy(hfg_ctx* c) {
     c->d_gcnt.release();
     c->d_gc_p0.release();
     c->d_one_sh.release();
-    c->d_eri_prog.release();
     c->d_eri_out.release();
     c->d_one_pairs.release();
     c->d_one_alpha.release();

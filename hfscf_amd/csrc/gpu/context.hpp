@@ -219,11 +219,7 @@ struct hfg_ctx {
     DevBuf<int> d_goff, d_gcnt;
     DevBuf<double4> d_gc_p0, d_gc_p1, d_gc_K;
     hfg::GcArgs gc_args{};  // device pointers of the group tables
-    // ERI export (hfg_eri_vector): cooperative programs of every class, made at the first call
-    std::vector<hfg::CoopArgs> eri_prog;
-    std::vector<size_t> eri_lds;
-    DevBuf<int4> d_eri_prog;
-    DevBuf<double> d_eri_out;
+    DevBuf<double> d_eri_out;  // the packed vector of hfg_eri_vector (freed after each call)
     // one-electron integrals (hfg_oneint, hfg_oneint_gradient)
     DevBuf<hfg::OneShell> d_one_sh;
     DevBuf<int2> d_one_pairs;

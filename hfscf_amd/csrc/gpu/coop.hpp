@@ -318,7 +318,7 @@ __global__ __launch_bounds__(kCoopBlock) void jk_coop_kernel(JkArgs a, CoopArgs 
                 I = src;
             }
             __syncthreads();
-            if (p.eri_out) {
+            if (a.eri_out) {
                 // export: every canonical element (i >= j, k >= l, ij >= kl; hfscf's index_ijkl,
                 // hfscf: src/math/hfscf_math.hpp:90) of the quartet, once. Each unordered pair of
                 // shell pairs is one quartet of the lists, so only a shell paired with itself (i < j
@@ -331,7 +331,7 @@ __global__ __launch_bounds__(kCoopBlock) void jk_coop_kernel(JkArgs a, CoopArgs 
                     if ((same_ab && fi < fj) || (same_cd && fk < fl)) continue;
                     const long long ij = tri(fi, fj), kl = tri(fk, fl);
                     if (same_q && ij < kl) continue;
-                    p.eri_out[tri(ij, kl)] = (double)I[o];
+                    a.eri_out[tri(ij, kl)] = (double)I[o];
                 }
                 __syncthreads();  // the next quartet reuses every LDS region
                 continue;
@@ -378,7 +378,7 @@ __global__ __launch_bounds__(kCoopBlock) void jk_coop_kernel(JkArgs a, CoopArgs 
             __syncthreads();  // the next quartet reuses every LDS region
         }
         __syncthreads();
-        if (p.eri_out) continue;  // an export has no tiles (the item may have no slab)
+        if (a.eri_out) continue;  // an export has no tiles (the item may have no slab)
         double* out = a.slab + w.slab;
         for (int k = lane; k < ntile; k += nl) out[k] = (double)tiles[k];
     }

@@ -45,9 +45,6 @@ struct CoopArgs {
     // LDS layout in R units after the tiles: region H (VRR table, then the HRR tables, then the
     // transform temporaries), the bra HRR output, the Cartesian block, the density blocks
     int off_bra, off_I, off_D, lds_r;
-    // ERI export (hfg_eri_vector): when set, each quartet's output-basis block is written to this
-    // packed vector at hfscf's canonical index instead of being digested into J/K tiles
-    double* eri_out;
 };
 
 constexpr int kCoopGradBlock = 256;  // four wavefronts share a quartet of the gradient kernel

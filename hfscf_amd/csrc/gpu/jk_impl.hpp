@@ -17,6 +17,28 @@ HFG_INLINE double density_bound(const PairDev& bp, const PairDev& kp, const floa
 
 constexpr int kChunk = 1024;  // candidate quartets screened per pass when density screening is on
 
+// Writes every canonical element (i >= j, k >= l, ij >= kl; hfscf's index_ijkl, hfscf:
+// src/math/hfscf_math.hpp:90) of one quartet's output-basis block I to the packed vector, once:
+// each unordered pair of shell pairs is one quartet of the lists, so only a shell paired with
+// itself (i < j within it) or a shell pair paired with itself (ij < kl) repeats an element.
+template <int NA, int NB, int NC, int ND, typename R>
+HFG_INLINE void export_block(const R* I, const PairDev& bp, const PairDev& kp, bool same_pair, double* out) {
+    auto tri = [](long long x, long long y) { return x >= y ? x * (x + 1) / 2 + y : y * (y + 1) / 2 + x; };
+    const bool same_ab = bp.sa == bp.sb, same_cd = kp.sa == kp.sb;
+    sfor<NA>([&](auto p) {
+        sfor<NB>([&](auto q) {
+            sfor<NC>([&](auto r) {
+                sfor<ND>([&](auto u) {
+                    const int fi = bp.ao_a + p, fj = bp.ao_b + q, fk = kp.ao_a + r, fl = kp.ao_b + u;
+                    const long long ij = tri(fi, fj), kl = tri(fk, fl);
+                    if (!((same_ab && fi < fj) || (same_cd && fk < fl) || (same_pair && ij < kl)))
+                        out[tri(ij, kl)] = (double)I[((p * NB + q) * NC + r) * ND + u];
+                });
+            });
+        });
+    });
+}
+
 template <int LA, int LB, int LC, int LD, bool PURE, typename R, typename ACC>
 __global__ __launch_bounds__(256) void jk_kernel(JkArgs a) {
     constexpr int NA = PURE && LA > 0 ? 2 * LA + 1 : ncart(LA);
@@ -107,6 +129,10 @@ __global__ __launch_bounds__(256) void jk_kernel(JkArgs a) {
                 R I[NA * NB * NC * ND];
                 eri_block<LA, LB, LC, LD, PURE, R>(bp, kp, a.pp0, a.pp1, a.pp2, a.boys, a.prim_cut, a.psq, a.elem_tol,
                                                    I);
+                if (a.eri_out) {  // export instead of digestion (uniform over the launch)
+                    export_block<NA, NB, NC, ND, R>(I, bp, kp, same && i == j, a.eri_out);
+                    continue;
+                }
                 if constexpr (kDeferJab) {
                     if (i != jab_row) {
                         flush_jab();
@@ -122,6 +148,7 @@ __global__ __launch_bounds__(256) void jk_kernel(JkArgs a) {
         }
         flush_jab();
         __syncthreads();
+        if (a.eri_out) continue;  // an export has no tiles (the item may have no slab)
         double* out = a.slab + w.slab;
         for (int k = threadIdx.x; k < ntile; k += blockDim.x) out[k] = (double)tiles[k];
     }

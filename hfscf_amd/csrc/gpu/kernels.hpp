@@ -31,6 +31,9 @@ struct JkArgs {
     const float* dmax;  // per molecule nshell x nshell at MolDev::dm_off
     double dtol;
     int* queue;         // work-item counter of this launch (zeroed before the launch)
+    // ERI export (hfg_eri_vector): when set, every quartet's output-basis block is written to
+    // this packed vector at hfscf's canonical index instead of being digested (no tiles)
+    double* eri_out;
 };
 
 struct GatherArgs {

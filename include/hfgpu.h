@@ -200,7 +200,7 @@ int hfg_oneint_gradient(hfg_ctx* ctx, const double* D, const double* Q, double* 
  * max(x,y)(max(x,y)+1)/2 + min(x,y), count = npair (npair + 1) / 2 with npair = nbf (nbf + 1) / 2,
  * in the output basis, with the context's screening (quartets and elements hfscf's screen drops
  * stay 0; its Cartesian diagonal), as hfscf stores them for post-SCF methods and the AO -> MO
- * transform. Replaces: the stored repulsion vector of hf_solver (hfscf: src/hfscf/hfscf.cpp:76,
+ * transform. The J/K kernels compute it (FP64) and write each element instead of digesting it. Replaces: the stored repulsion vector of hf_solver (hfscf: src/hfscf/hfscf.cpp:76,
  * src/basis/hfscf_trans_basis.cpp:7, src/math/hfscf_math.hpp:90; built by
  * src/integrals/hfscf_oseri.cpp:22-246). */
 int hfg_eri_vector(hfg_ctx* ctx, double* out, long long count);
