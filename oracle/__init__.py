@@ -4,7 +4,8 @@ Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import this
 
 * ``liboracle.so`` (ref_oracle.cpp): C++ restatement of hfscf's two-electron algorithm
   (chained 1-D Obara-Saika ERIs with incomplete-gamma Boys, mean-Schwarz screening, stored
-  ERIs, conventional digestion, on-the-fly derivative-ERI gradient contraction).
+  ERIs, conventional digestion, on-the-fly derivative-ERI gradient contraction, and the
+  second-derivative ERIs of the Hessian contracted with the density).
 * ``md_eri.py``: an independent McMurchie-Davidson ERI evaluator in numpy (a textbook
   scheme with no code in common with either the oracle or the GPU kernels).
 * ``scf_ref.py``: hfscf's RHF loop semantics (CORE guess, DIIS with max-error eviction,
@@ -53,6 +54,9 @@ def lib():
         L.orc_shell_quartet.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, _dp]
         L.orc_shell_quartet_deriv1.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, _dp]
         L.orc_grad2e.argtypes = [C.c_void_p, _dp, C.c_int, _dp]
+        L.orc_grad2e_parts.argtypes = [C.c_void_p, _dp, C.c_int, _dp, _dp]
+        L.orc_shell_quartet_deriv2.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, _dp]
+        L.orc_hess2e.argtypes = [C.c_void_p, _dp, C.c_int, _dp, _dp]
         L.orc_boys.argtypes = [C.c_int, C.c_double]
         L.orc_boys.restype = C.c_double
         L.orc_set_threads.argtypes = [C.c_int]
@@ -144,6 +148,26 @@ class Oracle:
         g = np.zeros((self.natom, 3))
         lib().orc_grad2e(self.h, _p(np.ascontiguousarray(D)), self.natom, _p(g))
         return g
+
+    def grad2e_parts(self, D):
+        """The Coulomb and exchange parts of grad2e (hfscf's gradCoulomb, gradExchange)."""
+        gc, gx = np.zeros((self.natom, 3)), np.zeros((self.natom, 3))
+        lib().orc_grad2e_parts(self.h, _p(np.ascontiguousarray(D)), self.natom, _p(gc), _p(gx))
+        return gc, gx
+
+    def shell_quartet_deriv2(self, a, b, c, d):
+        """45 Cartesian blocks, one per coordinate pair X1 <= X2 of (A, B, C) x (x, y, z)."""
+        m = [(int(self.L[s]) + 1) * (int(self.L[s]) + 2) // 2 for s in (a, b, c, d)]
+        out = np.zeros([45] + m)
+        lib().orc_shell_quartet_deriv2(self.h, a, b, c, d, _p(out))
+        return out
+
+    def hess2e(self, D):
+        """Two-electron Hessian at fixed density: (HC, HX), hfscf's m_hessianC and m_hessianEx."""
+        n3 = 3 * self.natom
+        hc, hx = np.zeros((n3, n3)), np.zeros((n3, n3))
+        lib().orc_hess2e(self.h, _p(np.ascontiguousarray(D)), self.natom, _p(hc), _p(hx))
+        return hc, hx
 
 
 def boys(m, T):

@@ -22,6 +22,12 @@
 //                                    src/integrals/hfscf_oserideriv1.cpp:20-374, contracted on
 //                                    the fly with the RHF two-particle density of
 //                                    src/gradient/hfscf_grad_rhf.cpp:121-141 (no stored tensors)
+//   * second-derivative ERIs by applying the shift rule twice (d2/dA_x dA_y on one centre:
+//     4a^2 (a+1x+1y) - 2a (a+1x)_y (a+1x-1y) - 2a a_x (a-1x+1y) + a_x (a-1x)_y (a-1x-1y)), D by
+//     translation, contracted with the Coulomb and exchange densities of hfscf's
+//     calc_scf_hessian_tei (src/hessian/hfscf_hessian_ints.cpp:20-313); hfscf evaluates the
+//     same derivatives with its chained recursions (src/integrals/hfscf_oserideriv2qq.cpp:28-749,
+//     hfscf_oserideriv2pq.cpp:23-799) and stores them as N^4 tensors per direction pair
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
@@ -29,6 +35,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <vector>
 
 #ifdef _OPENMP
@@ -269,8 +276,8 @@ inline int64_t index_ijkl(int64_t i, int64_t j, int64_t k, int64_t l) {
 
 // Contracted Cartesian quartet by the chained 1-D scheme (hfscf_oseri.cpp:48-200).
 // Angular momenta are given explicitly so the same routine also evaluates the shifted
-// classes needed for derivatives; wmode selects a per-primitive weight
-// (0: 1, 1: 2 alpha_A, 2: 2 alpha_B, 3: 2 alpha_C). skip_mode: 0 = compute every
+// classes needed for derivatives; wmode selects a per-primitive weight, the product of
+// the factors in its 2-bit fields (0: 1, 1: 2 alpha_A, 2: 2 alpha_B, 3: 2 alpha_C). skip_mode: 0 = compute every
 // component, 1 = hfscf's Cartesian permutation skips (idx1 > idx2, idx3 > idx4),
 // 2 = additionally hfscf's element Schwarz skip and (pq|pq) taken from the screen.
 // The per-component seed cutoff uses `seed_cut`; the primitive cutoff `prim_cut`.
@@ -333,9 +340,11 @@ void cart_quartet(const Oracle& o, const Pair& s12, const Pair& s34, const int L
                     const double pr = eta / gab, qr = eta / gcd;
                     double pfac = s12.K[id12] * s34.K[id34] * kPI25 / (gab * gcd * std::sqrt(gab + gcd));
                     if (std::fabs(pfac) < prim_cut) continue;
-                    if (wmode == 1) pfac *= 2 * A.a[p1];
-                    if (wmode == 2) pfac *= 2 * B.a[p2];
-                    if (wmode == 3) pfac *= 2 * C.a[p3];
+                    for (int wm = wmode; wm; wm >>= 2) {  // up to two factors, 2 bits each
+                        if ((wm & 3) == 1) pfac *= 2 * A.a[p1];
+                        if ((wm & 3) == 2) pfac *= 2 * B.a[p2];
+                        if ((wm & 3) == 3) pfac *= 2 * C.a[p3];
+                    }
                     const double T = eta * rpq2, eT = std::exp(-T);
                     if (std::fabs(rpq2) > rpq_exact) {
                         Jx(0, 0, 0, 0, nmax) = boys_ref(nmax, T);
@@ -762,9 +771,22 @@ void orc_shell_quartet_deriv1(void* h, int i, int j, int k, int l, double* out) 
     }
 }
 
+void grad2e_w(void* h, const double* D, int natom, double* grad, double wc, double wx);
+
 // Two-electron RHF gradient sum_pqrs [2 D_pq D_rs - D_pr D_qs] (pq|rs)', contracted on the
 // fly over canonical shell quartets (natom x 3 added into grad).
-void orc_grad2e(void* h, const double* D, int natom, double* grad) {
+void orc_grad2e(void* h, const double* D, int natom, double* grad) { grad2e_w(h, D, natom, grad, 1.0, 1.0); }
+
+// The same with the Coulomb part weighted by wc and the exchange part by wx (wc = 1, wx = 0:
+// hfscf's gradCoulomb; wc = 0, wx = 1: its gradExchange, hfscf_grad_rhf.cpp:140-141).
+void orc_grad2e_parts(void* h, const double* D, int natom, double* gradC, double* gradX) {
+    grad2e_w(h, D, natom, gradC, 1.0, 0.0);
+    grad2e_w(h, D, natom, gradX, 0.0, 1.0);
+}
+
+}  // extern "C"
+
+void grad2e_w(void* h, const double* D, int natom, double* grad, double wc, double wx) {
     Oracle& o = *(Oracle*)h;
     const int ns = o.nsh();
     const int n = o.nbf();
@@ -812,9 +834,9 @@ void orc_grad2e(void* h, const double* D, int natom, double* grad) {
                                     for (int r = 0; r < no[2]; ++r)
                                         for (int t = 0; t < no[3]; ++t) {
                                             const int P_ = off[0] + p, Q_ = off[1] + q, R_ = off[2] + r, S_ = off[3] + t;
-                                            const double gam = 2.0 * D[(size_t)P_ * n + Q_] * D[(size_t)R_ * n + S_] -
-                                                               0.5 * (D[(size_t)P_ * n + R_] * D[(size_t)Q_ * n + S_] +
-                                                                      D[(size_t)P_ * n + S_] * D[(size_t)Q_ * n + R_]);
+                                            const double gam = wc * 2.0 * D[(size_t)P_ * n + Q_] * D[(size_t)R_ * n + S_] -
+                                                               wx * 0.5 * (D[(size_t)P_ * n + R_] * D[(size_t)Q_ * n + S_] +
+                                                                           D[(size_t)P_ * n + S_] * D[(size_t)Q_ * n + R_]);
                                             s += gam * use[(((size_t)p * no[1] + q) * no[2] + r) * no[3] + t];
                                         }
                             (void)nbo;
@@ -827,6 +849,215 @@ void orc_grad2e(void* h, const double* D, int natom, double* grad) {
         for (size_t k = 0; k < total.size(); ++k) total[k] += g[k];
     }
     for (size_t k = 0; k < total.size(); ++k) grad[k] += total[k];
+}
+
+// Cartesian second-derivative blocks of one quartet: 45 blocks of ncart^4, one per coordinate
+// pair (X1 <= X2) of X = 3 * centre + direction over the centres A, B, C, in the order
+// (0,0) (0,1) .. (0,8) (1,1) .. (8,8).
+void quartet_deriv2(Oracle& o, int i, int j, int k, int l, std::vector<double>& out) {
+    QuartetWork w;
+    const int L[4] = {o.sh[i].L, o.sh[j].L, o.sh[k].L, o.sh[l].L};
+    int m[4];
+    for (int t = 0; t < 4; ++t) m[t] = (L[t] + 1) * (L[t] + 2) / 2;
+    const size_t nb = (size_t)m[0] * m[1] * m[2] * m[3];
+    out.assign(45 * nb, 0.0);
+    auto cidx = [](const int e[3]) {
+        const int ii = e[1] + e[2];
+        return ii * (ii + 1) / 2 + e[2];
+    };
+    auto comp = [](int l, int c, int e[3]) {
+        int ci = 0;
+        for (int a = 0; a <= l; ++a)
+            for (int b = 0; b <= a; ++b, ++ci)
+                if (ci == c) {
+                    e[0] = l - a;
+                    e[1] = a - b;
+                    e[2] = b;
+                    return;
+                }
+    };
+    // shifted classes by (angular momentum shifts, weight mode), evaluated once per quartet
+    std::map<int, std::vector<double>> cache;
+    auto cls = [&](const int dL[4], int wmode) -> const std::vector<double>& {
+        int key = wmode;
+        for (int t = 0; t < 4; ++t) key = key * 5 + (dL[t] + 2);
+        auto it = cache.find(key);
+        if (it != cache.end()) return it->second;
+        int Lp[4];
+        for (int t = 0; t < 4; ++t) Lp[t] = L[t] + dL[t];
+        std::vector<double>& v = cache[key];
+        cart_quartet(o, o.pair(i, j), o.pair(k, l), Lp, wmode, 0, o.cutoff / 10000.0, 0.0, false, 1e-18, v, w);
+        return v;
+    };
+    auto at = [&](const std::vector<double>& v, const int dL[4], const int c4[4]) {
+        int mp[4];
+        for (int t = 0; t < 4; ++t) mp[t] = (L[t] + dL[t] + 1) * (L[t] + dL[t] + 2) / 2;
+        return v[(((size_t)c4[0] * mp[1] + c4[1]) * mp[2] + c4[2]) * mp[3] + c4[3]];
+    };
+    int pidx = 0;
+    for (int X1 = 0; X1 < 9; ++X1)
+        for (int X2 = X1; X2 < 9; ++X2, ++pidx) {
+            const int c1 = X1 / 3, d1 = X1 % 3, c2 = X2 / 3, d2 = X2 % 3;
+            for (int a = 0; a < m[0]; ++a)
+                for (int b = 0; b < m[1]; ++b)
+                    for (int c = 0; c < m[2]; ++c)
+                        for (int d = 0; d < m[3]; ++d) {
+                            const int ci[4] = {a, b, c, d};
+                            double val = 0.0;
+                            if (c1 != c2) {
+                                // one shift on each of two centres: (2a+ - e-) x (2b+ - e-)
+                                int e1[3] = {0, 0, 0}, e2[3] = {0, 0, 0};
+                                comp(L[c1], ci[c1], e1);
+                                comp(L[c2], ci[c2], e2);
+                                for (int s1 = 1; s1 >= -1; s1 -= 2)
+                                    for (int s2 = 1; s2 >= -1; s2 -= 2) {
+                                        const double f = (s1 > 0 ? 1.0 : -e1[d1]) * (s2 > 0 ? 1.0 : -e2[d2]);
+                                        if (f == 0.0) continue;
+                                        int dL[4] = {0, 0, 0, 0};
+                                        dL[c1] = s1;
+                                        dL[c2] = s2;
+                                        const int wmode = (s1 > 0 ? c1 + 1 : 0) | ((s2 > 0 ? c2 + 1 : 0) << 2);
+                                        int f1[3] = {e1[0], e1[1], e1[2]}, f2[3] = {e2[0], e2[1], e2[2]};
+                                        f1[d1] += s1;
+                                        f2[d2] += s2;
+                                        int c4[4] = {a, b, c, d};
+                                        c4[c1] = cidx(f1);
+                                        c4[c2] = cidx(f2);
+                                        val += f * at(cls(dL, wmode), dL, c4);
+                                    }
+                            } else {
+                                // two shifts on one centre
+                                int e[3] = {0, 0, 0};
+                                comp(L[c1], ci[c1], e);
+                                for (int s1 = 1; s1 >= -1; s1 -= 2)
+                                    for (int s2 = 1; s2 >= -1; s2 -= 2) {
+                                        int f1[3] = {e[0], e[1], e[2]};
+                                        double f;
+                                        if (s1 > 0) {
+                                            f1[d1] += 1;
+                                            f = s2 > 0 ? 1.0 : -(double)f1[d2];
+                                        } else {
+                                            f = -(double)e[d1];
+                                            f1[d1] -= 1;
+                                            if (s2 < 0) f *= -(double)f1[d2];
+                                        }
+                                        if (f == 0.0) continue;
+                                        f1[d2] += s2;
+                                        int dL[4] = {0, 0, 0, 0};
+                                        dL[c1] = s1 + s2;
+                                        const int wmode = (s1 > 0 ? c1 + 1 : 0) | ((s2 > 0 ? c1 + 1 : 0) << 2);
+                                        int c4[4] = {a, b, c, d};
+                                        c4[c1] = cidx(f1);
+                                        val += f * at(cls(dL, wmode), dL, c4);
+                                    }
+                            }
+                            out[(size_t)pidx * nb + (((size_t)a * m[1] + b) * m[2] + c) * m[3] + d] = val;
+                        }
+        }
+}
+
+extern "C" {
+
+void orc_shell_quartet_deriv2(void* h, int i, int j, int k, int l, double* out) {
+    std::vector<double> v;
+    quartet_deriv2(*(Oracle*)h, i, j, k, l, v);
+    std::memcpy(out, v.data(), v.size() * sizeof(double));
+}
+
+// Two-electron part of the RHF Hessian with the density held fixed, split as hfscf's
+// m_hessianC = sum D_ij D_kl 2 (ij|kl)'' and m_hessianEx = -sum D_ij D_kl (ik|jl)''
+// (hfscf_hessian_ints.cpp:20-313), both (3 natom) x (3 natom), added into HC and HX. Canonical
+// shell quartets with their degeneracy, the 8-fold symmetrised densities of the gradient, the
+// D-centre rows and columns from translational invariance (d/dD = -(d/dA + d/dB + d/dC)).
+void orc_hess2e(void* h, const double* D, int natom, double* HC, double* HX) {
+    Oracle& o = *(Oracle*)h;
+    const int ns = o.nsh();
+    const int n = o.nbf();
+    const int n3 = 3 * natom;
+    std::vector<double> totC((size_t)n3 * n3, 0.0), totX((size_t)n3 * n3, 0.0);
+#pragma omp parallel
+    {
+        std::vector<double> hc((size_t)n3 * n3, 0.0), hx((size_t)n3 * n3, 0.0);
+        std::vector<double> der, pblk, cblk;
+#pragma omp for schedule(dynamic)
+        for (int a = 0; a < ns; ++a)
+            for (int b = 0; b <= a; ++b)
+                for (int c = 0; c <= a; ++c)
+                    for (int d = 0; d <= c; ++d) {
+                        if (index_ij(a, b) < index_ij(c, d)) continue;
+                        double deg = 8.0;
+                        if (a == b) deg *= 0.5;
+                        if (c == d) deg *= 0.5;
+                        if (a == c && b == d) deg *= 0.5;
+                        const int L[4] = {o.sh[a].L, o.sh[b].L, o.sh[c].L, o.sh[d].L};
+                        int m[4], no[4], off[4];
+                        for (int t = 0; t < 4; ++t) {
+                            m[t] = (L[t] + 1) * (L[t] + 2) / 2;
+                            no[t] = o.pure ? 2 * L[t] + 1 : m[t];
+                        }
+                        const int sh4[4] = {a, b, c, d};
+                        for (int t = 0; t < 4; ++t) off[t] = o.pure ? o.sh[sh4[t]].ids : o.sh[sh4[t]].idx;
+                        const size_t nb = (size_t)m[0] * m[1] * m[2] * m[3];
+                        quartet_deriv2(o, a, b, c, d, der);
+                        double MC[9][9], MX[9][9];
+                        int pidx = 0;
+                        for (int X1 = 0; X1 < 9; ++X1)
+                            for (int X2 = X1; X2 < 9; ++X2, ++pidx) {
+                                const double* blk = der.data() + pidx * nb;
+                                const double* use = blk;
+                                if (o.pure && (L[0] + L[1] + L[2] + L[3])) {
+                                    cblk.assign(blk, blk + nb);
+                                    to_pure(o, L, cblk, pblk);
+                                    use = pblk.data();
+                                }
+                                double sc = 0, sx = 0;
+                                for (int p = 0; p < no[0]; ++p)
+                                    for (int q = 0; q < no[1]; ++q)
+                                        for (int r = 0; r < no[2]; ++r)
+                                            for (int t = 0; t < no[3]; ++t) {
+                                                const int P_ = off[0] + p, Q_ = off[1] + q, R_ = off[2] + r, S_ = off[3] + t;
+                                                const double v = use[(((size_t)p * no[1] + q) * no[2] + r) * no[3] + t];
+                                                sc += 2.0 * D[(size_t)P_ * n + Q_] * D[(size_t)R_ * n + S_] * v;
+                                                sx -= 0.5 * (D[(size_t)P_ * n + R_] * D[(size_t)Q_ * n + S_] +
+                                                             D[(size_t)P_ * n + S_] * D[(size_t)Q_ * n + R_]) * v;
+                                            }
+                                MC[X1][X2] = MC[X2][X1] = deg * sc;
+                                MX[X1][X2] = MX[X2][X1] = deg * sx;
+                            }
+                        // 12 x 12 over A, B, C, D: T M T^T with the D rows -(A + B + C)
+                        const int atoms[4] = {o.sh[a].atom, o.sh[b].atom, o.sh[c].atom, o.sh[d].atom};
+                        auto tcoef = [](int Y, int X) {  // T[Y][X], Y over 12, X over 9
+                            if (Y < 9) return Y == X ? 1.0 : 0.0;
+                            return (X % 3 == Y % 3) ? -1.0 : 0.0;
+                        };
+                        for (int Y1 = 0; Y1 < 12; ++Y1)
+                            for (int Y2 = 0; Y2 < 12; ++Y2) {
+                                double vc = 0, vx = 0;
+                                for (int X1 = 0; X1 < 9; ++X1) {
+                                    const double t1 = tcoef(Y1, X1);
+                                    if (t1 == 0.0) continue;
+                                    for (int X2 = 0; X2 < 9; ++X2) {
+                                        const double t2 = tcoef(Y2, X2);
+                                        if (t2 == 0.0) continue;
+                                        vc += t1 * t2 * MC[X1][X2];
+                                        vx += t1 * t2 * MX[X1][X2];
+                                    }
+                                }
+                                const size_t r = (size_t)atoms[Y1 / 3] * 3 + Y1 % 3, col = (size_t)atoms[Y2 / 3] * 3 + Y2 % 3;
+                                hc[r * n3 + col] += vc;
+                                hx[r * n3 + col] += vx;
+                            }
+                    }
+#pragma omp critical
+        for (size_t q = 0; q < totC.size(); ++q) {
+            totC[q] += hc[q];
+            totX[q] += hx[q];
+        }
+    }
+    for (size_t q = 0; q < totC.size(); ++q) {
+        HC[q] += totC[q];
+        HX[q] += totX[q];
+    }
 }
 
 double orc_boys(int m, double T) { return boys_ref(m, T); }
