@@ -102,6 +102,7 @@ def hfgpu():
         lib.hfg_energy_backward.argtypes = [C.c_void_p, _dp, _dp]
         lib.hfg_oneint.argtypes = [C.c_void_p, _dp, _dp, _dp]
         lib.hfg_eri_vector.argtypes = [C.c_void_p, _dp, C.c_longlong]
+        lib.hfg_hessian_2e.argtypes = [C.c_void_p, _dp, _dp, _dp]
         lib.hfg_oneint_gradient.argtypes = [C.c_void_p, _dp, _dp, _dp]
         lib.hfg_eri_shell_quartet.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, _dp]
         lib.hfg_schwarz.argtypes = [C.c_void_p, _dp]
@@ -542,6 +543,17 @@ class GpuContext:
         v = np.zeros(npair * (npair + 1) // 2)
         self._check(hfgpu().hfg_eri_vector(self._h, _ptr(v), len(v)))
         return v
+
+    def hessian_2e(self, D):
+        """Two-electron Hessian at the fixed density D: (HC, HX), each (3 natom) x (3 natom), as
+        hfscf's m_hessianC and m_hessianEx (single-molecule contexts, shells up to d)."""
+        n3 = 3 * self.natom
+        hc, hx = np.zeros((n3, n3)), np.zeros((n3, n3))
+        D = np.ascontiguousarray(D, dtype=np.float64)
+        if D.shape != (self.nbf, self.nbf):
+            raise ValueError(f"hessian_2e: D must be {self.nbf} x {self.nbf}")
+        self._check(hfgpu().hfg_hessian_2e(self._h, _ptr(D), _ptr(hc), _ptr(hx)))
+        return hc, hx
 
     def eri_block(self, s1, s2, s3, s4):
         sh = self.mol.shells()

@@ -1,0 +1,489 @@
+// Two-electron part of the RHF Hessian at a fixed density on the GPU: hfscf's
+//   m_hessianC(X1, X2)  =  sum_ijkl D_ij D_kl 2 d2(ij|kl)/dX1 dX2
+//   m_hessianEx(X1, X2) = -sum_ijkl D_ij D_kl   d2(ik|jl)/dX1 dX2
+// (hfscf: src/hessian/hfscf_hessian_ints.cpp:20-313). hfscf evaluates the second-derivative
+// integrals with its chained recursions into six N^4 tensors per direction pair
+// (src/integrals/hfscf_oserideriv2qq.cpp:28-749, hfscf_oserideriv2pq.cpp:23-799) and contracts
+// them afterwards; here they are contracted with the density as they are made and never stored.
+//
+// One wavefront per canonical shell quartet (ab|cd), a >= b, c >= d, (ab) >= (cd). Integrals by
+// McMurchie-Davidson: per primitive quartet the 1-D Hermite coefficients E^{ij}_t of the bra
+// (i <= la + 2, j <= lb + 2) and ket (i <= lc + 2, j <= ld) in LDS, and the Hermite Coulomb
+// integrals R_tuv(pq/(p+q), P - Q) up to order la + lb + lc + ld + 2, made level by level by the
+// wavefront. A derivative of a Gaussian with respect to its centre is d/dA_x |a> = 2 alpha |a+1x>
+// - a_x |a-1x>, applied once or twice to the centres A, B, C (D from translational invariance:
+// d/dD = -(d/dA + d/dB + d/dC)); it acts on one direction's Hermite coefficients only, so each
+// lane, holding one Cartesian component quartet, forms per direction the convolution over the
+// bra and ket Hermite indices of the shifted coefficients (ten bra/ket operator combinations of
+// total order <= 2) and sums them against R for all 45 coordinate pairs of (A, B, C) x (x, y, z).
+// The lane adds each pair's integral times its Coulomb and exchange densities into 90
+// accumulators; at the quartet's end the wavefront reduces them, expands the 9 x 9 block to the
+// 12 x 12 block of the four centres and adds it to the atoms' rows and columns.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <vector>
+
+#include "boys.hpp"
+#include "common.hpp"
+#include "context.hpp"
+#include "coop.hpp"
+#include "oneint.hpp"
+
+namespace hfg {
+
+namespace {
+
+constexpr int kHL = 2;                    // largest shell angular momentum
+constexpr int kHE = kHL + 3;              // exponents 0 .. L + 2 of a shifted centre
+constexpr int kHT = 2 * kHL + 5;          // Hermite index t <= i + j <= 2 L + 4
+constexpr int kHKet = kHL + 1;            // ket j (centre D, never shifted) <= L
+constexpr int kHN = 4 * kHL + 2;          // R_tuv order <= L_total + 2
+constexpr int kHR = (kHN + 1) * (kHN + 2) * (kHN + 3) / 6;
+constexpr int kHS = kHN + 1;              // convolution length of one direction
+constexpr int kHBlock = 64;               // one wavefront per shell quartet
+constexpr int kHPairs = 45;               // coordinate pairs X1 <= X2 over (A, B, C) x (x, y, z)
+
+// (t, u, v) of the packed R index ridx(t, u, v) = s(s+1)(s+2)/6 + (s-t)(s-t+1)/2 + v, s = t+u+v.
+struct TuvTab {
+    unsigned char t[kHR], u[kHR], v[kHR];
+};
+constexpr TuvTab make_tuv() {
+    TuvTab r{};
+    int n = 0;
+    for (int s = 0; s <= kHN; ++s)
+        for (int q = 0; q <= s; ++q)
+            for (int v = 0; v <= q; ++v, ++n) {
+                r.t[n] = (unsigned char)(s - q);
+                r.u[n] = (unsigned char)(q - v);
+                r.v[n] = (unsigned char)v;
+            }
+    return r;
+}
+__constant__ TuvTab kTuv = make_tuv();
+
+__host__ __device__ constexpr int ridx(int t, int u, int v) {
+    const int s = t + u + v;
+    return s * (s + 1) * (s + 2) / 6 + (s - t) * (s - t + 1) / 2 + v;
+}
+__host__ __device__ constexpr int rsize(int n) { return (n + 1) * (n + 2) * (n + 3) / 6; }
+
+// Operators on one direction. Bra codes: 0 none, 1 d/dA, 2 d/dB, 3 d2/dA2, 4 d2/dA dB, 5 d2/dB2;
+// ket codes: 0 none, 1 d/dC, 2 d2/dC2. The ten combinations of total order <= 2:
+constexpr int kComboBra[10] = {0, 1, 2, 3, 4, 5, 0, 1, 2, 0};
+constexpr int kComboKet[10] = {0, 0, 0, 0, 0, 0, 1, 1, 1, 2};
+constexpr int kBraOrder[6] = {0, 1, 1, 2, 2, 2};
+
+// Combination per direction of coordinate pair `op` (X1 <= X2, X = 3 centre + direction, in the
+// order (0,0) (0,1) .. (0,8) (1,1) .. (8,8)).
+struct PairDirs {
+    int k[3];
+    int x1, x2;
+};
+constexpr PairDirs pair_dirs(int op) {
+    int X1 = 0, X2 = 0, n = 0;
+    for (int a = 0; a < 9; ++a)
+        for (int b = a; b < 9; ++b, ++n)
+            if (n == op) {
+                X1 = a;
+                X2 = b;
+            }
+    int bc[3] = {0, 0, 0}, kc[3] = {0, 0, 0};
+    const int xs[2] = {X1, X2};
+    for (int q = 0; q < 2; ++q) {
+        const int c = xs[q] / 3, d = xs[q] % 3;
+        if (c == 0)
+            bc[d] = bc[d] == 0 ? 1 : (bc[d] == 1 ? 3 : 4);
+        else if (c == 1)
+            bc[d] = bc[d] == 0 ? 2 : (bc[d] == 2 ? 5 : 4);
+        else
+            kc[d] += 1;
+    }
+    PairDirs r{{0, 0, 0}, X1, X2};
+    for (int d = 0; d < 3; ++d) r.k[d] = kc[d] == 0 ? bc[d] : (kc[d] == 2 ? 9 : (bc[d] == 0 ? 6 : (bc[d] == 1 ? 7 : 8)));
+    return r;
+}
+
+struct HessArgs {
+    const OneShell* sh;
+    const double* alpha;
+    const double* coef;
+    const double* xyz;
+    const double* boys;
+    const int2* pairs;    // canonical shell pairs (a, b), a >= b
+    long long nquart;     // npairs (npairs + 1) / 2 canonical quartets
+    const int* cart_off;  // first Cartesian function of each shell
+    int ncart, natom;
+    const double* Dc;     // ncart x ncart density in the Cartesian basis
+    double prim_cut;      // primitive quartets with |prefactor| below it are skipped
+    double* HC;           // (3 natom)^2 Coulomb part (zeroed before the launch)
+    double* HX;           // (3 natom)^2 exchange part
+};
+
+__device__ inline void cart_exps(int L, int k, int* e) {
+    int n = 0;
+    for (int i = 0; i <= L; ++i)
+        for (int j = 0; j <= i; ++j, ++n)
+            if (n == k) {
+                e[0] = L - i;
+                e[1] = i - j;
+                e[2] = j;
+            }
+}
+
+// E^{ij}_t of one direction at E[(i * sj + j) * kHT + t]; zero outside 0 <= t <= i + j.
+__device__ inline double eget(const double* E, int sj, int i, int j, int t) {
+    return (i < 0 || j < 0 || t < 0 || t > i + j) ? 0.0 : E[(i * sj + j) * kHT + t];
+}
+
+__device__ void herm_fill(double* E, int sj, int imax, int jmax, double o2p, double xpa, double xpb) {
+    for (int i = 0; i <= imax; ++i)
+        for (int j = 0; j <= jmax; ++j)
+            for (int t = 0; t <= i + j; ++t) {
+                double v;
+                if (i == 0 && j == 0)
+                    v = 1.0;
+                else if (i > 0)
+                    v = o2p * eget(E, sj, i - 1, j, t - 1) + xpa * eget(E, sj, i - 1, j, t) +
+                        (t + 1) * eget(E, sj, i - 1, j, t + 1);
+                else
+                    v = o2p * eget(E, sj, i, j - 1, t - 1) + xpb * eget(E, sj, i, j - 1, t) +
+                        (t + 1) * eget(E, sj, i, j - 1, t + 1);
+                E[(i * sj + j) * kHT + t] = v;
+            }
+}
+
+// Shifted bra coefficient of code `code` (za = 2 alpha, zb = 2 beta).
+__device__ inline double bra_op(const double* E, int code, int a, int b, int t, double za, double zb) {
+    constexpr int sj = kHE;
+    switch (code) {
+        case 0: return eget(E, sj, a, b, t);
+        case 1: return za * eget(E, sj, a + 1, b, t) - a * eget(E, sj, a - 1, b, t);
+        case 2: return zb * eget(E, sj, a, b + 1, t) - b * eget(E, sj, a, b - 1, t);
+        case 3:
+            return za * za * eget(E, sj, a + 2, b, t) - za * (2 * a + 1) * eget(E, sj, a, b, t) +
+                   a * (a - 1) * eget(E, sj, a - 2, b, t);
+        case 4:
+            return za * zb * eget(E, sj, a + 1, b + 1, t) - za * b * eget(E, sj, a + 1, b - 1, t) -
+                   zb * a * eget(E, sj, a - 1, b + 1, t) + a * b * eget(E, sj, a - 1, b - 1, t);
+        default:
+            return zb * zb * eget(E, sj, a, b + 2, t) - zb * (2 * b + 1) * eget(E, sj, a, b, t) +
+                   b * (b - 1) * eget(E, sj, a, b - 2, t);
+    }
+}
+
+// Shifted ket coefficient (zc = 2 gamma).
+__device__ inline double ket_op(const double* E, int code, int c, int d, int t, double zc) {
+    constexpr int sj = kHKet;
+    switch (code) {
+        case 0: return eget(E, sj, c, d, t);
+        case 1: return zc * eget(E, sj, c + 1, d, t) - c * eget(E, sj, c - 1, d, t);
+        default:
+            return zc * zc * eget(E, sj, c + 2, d, t) - zc * (2 * c + 1) * eget(E, sj, c, d, t) +
+                   c * (c - 1) * eget(E, sj, c - 2, d, t);
+    }
+}
+
+__device__ inline void boys_upto(int N, double T, const double* tab, double* F) {
+    switch (N) {
+        case 2: boys_eval<2, double>(T, tab, 1.0, F); break;
+        case 3: boys_eval<3, double>(T, tab, 1.0, F); break;
+        case 4: boys_eval<4, double>(T, tab, 1.0, F); break;
+        case 5: boys_eval<5, double>(T, tab, 1.0, F); break;
+        case 6: boys_eval<6, double>(T, tab, 1.0, F); break;
+        case 7: boys_eval<7, double>(T, tab, 1.0, F); break;
+        case 8: boys_eval<8, double>(T, tab, 1.0, F); break;
+        case 9: boys_eval<9, double>(T, tab, 1.0, F); break;
+        default: boys_eval<kHN, double>(T, tab, 1.0, F); break;
+    }
+}
+
+__device__ inline double wave_sum(double v) {
+    for (int o = kHBlock / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kHBlock);
+    return v;
+}
+
+__global__ __launch_bounds__(kHBlock) void hess2e_kernel(HessArgs g) {
+    __shared__ double Eb[3 * kHE * kHE * kHT];
+    __shared__ double Ek[3 * kHE * kHKet * kHT];
+    __shared__ double Rb[2][kHR];
+    __shared__ double Fs[kHN + 1];
+    __shared__ double M[2][81];
+    const int lane = threadIdx.x;
+    for (long long q = blockIdx.x; q < g.nquart; q += gridDim.x) {
+        // quartet q -> canonical pair indices P >= Q
+        long long P = (long long)((sqrt(8.0 * (double)q + 1.0) - 1.0) * 0.5);
+        while (P * (P + 1) / 2 > q) --P;
+        while ((P + 1) * (P + 2) / 2 <= q) ++P;
+        const long long Qp = q - P * (P + 1) / 2;
+        const int2 pab = g.pairs[P], pcd = g.pairs[Qp];
+        const int s4[4] = {pab.x, pab.y, pcd.x, pcd.y};
+        const OneShell A = g.sh[s4[0]], B = g.sh[s4[1]], C = g.sh[s4[2]], D = g.sh[s4[3]];
+        const double* ra = g.xyz + 3 * A.atom;
+        const double* rb = g.xyz + 3 * B.atom;
+        const double* rc = g.xyz + 3 * C.atom;
+        const double* rd = g.xyz + 3 * D.atom;
+        double deg = 8.0;
+        if (pab.x == pab.y) deg *= 0.5;
+        if (pcd.x == pcd.y) deg *= 0.5;
+        if (P == Qp) deg *= 0.5;
+        const int na = (A.L + 1) * (A.L + 2) / 2, nb = (B.L + 1) * (B.L + 2) / 2;
+        const int nc = (C.L + 1) * (C.L + 2) / 2, nd = (D.L + 1) * (D.L + 2) / 2;
+        const int ncomp = na * nb * nc * nd;
+        const int N = A.L + B.L + C.L + D.L + 2;
+        const int oa = g.cart_off[s4[0]], ob = g.cart_off[s4[1]], oc = g.cart_off[s4[2]], od = g.cart_off[s4[3]];
+        double ab2 = 0.0, cd2 = 0.0;
+        for (int d = 0; d < 3; ++d) {
+            ab2 += (ra[d] - rb[d]) * (ra[d] - rb[d]);
+            cd2 += (rc[d] - rd[d]) * (rc[d] - rd[d]);
+        }
+        double accC[kHPairs], accX[kHPairs];
+        for (int k = 0; k < kHPairs; ++k) accC[k] = accX[k] = 0.0;
+        for (int i = 0; i < A.np; ++i)
+            for (int j = 0; j < B.np; ++j) {
+                const double al = g.alpha[A.po + i], be = g.alpha[B.po + j], p = al + be;
+                const double Kab = exp(-al * be / p * ab2) * g.coef[A.po + i] * g.coef[B.po + j];
+                double Pc[3];
+                for (int d = 0; d < 3; ++d) Pc[d] = (al * ra[d] + be * rb[d]) / p;
+                // the previous quartet's last use of Eb ended at a barrier
+                if (lane < 3)
+                    herm_fill(Eb + lane * kHE * kHE * kHT, kHE, A.L + 2, B.L + 2, 0.5 / p, Pc[lane] - ra[lane],
+                              Pc[lane] - rb[lane]);
+                for (int k = 0; k < C.np; ++k)
+                    for (int l = 0; l < D.np; ++l) {
+                        const double ga = g.alpha[C.po + k], de = g.alpha[D.po + l], qx = ga + de;
+                        const double Kcd = exp(-ga * de / qx * cd2) * g.coef[C.po + k] * g.coef[D.po + l];
+                        const double pref = 34.986836655249725693 * Kab * Kcd / (p * qx * sqrt(p + qx));  // 2 pi^2.5
+                        if (fabs(pref) < g.prim_cut) continue;  // uniform over the wavefront
+                        double Qc[3], PQ[3];
+                        for (int d = 0; d < 3; ++d) {
+                            Qc[d] = (ga * rc[d] + de * rd[d]) / qx;
+                            PQ[d] = Pc[d] - Qc[d];
+                        }
+                        const double aR = p * qx / (p + qx);
+                        if (lane < 3)
+                            herm_fill(Ek + lane * kHE * kHKet * kHT, kHKet, C.L + 2, D.L, 0.5 / qx, Qc[lane] - rc[lane],
+                                      Qc[lane] - rd[lane]);
+                        if (lane == 3) {
+                            double F[kHN + 1];
+                            boys_upto(N, aR * (PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]), g.boys, F);
+                            double f = 1.0;
+                            for (int m = 0; m <= N; ++m) {
+                                Fs[m] = f * F[m];
+                                f *= -2.0 * aR;
+                            }
+                        }
+                        __syncthreads();
+                        // R^(m)_tuv from order N down to 0; level m lives in Rb[m & 1]
+                        if (lane == 0) Rb[N & 1][0] = Fs[N];
+                        __syncthreads();
+                        for (int m = N - 1; m >= 0; --m) {
+                            const double* src = Rb[(m + 1) & 1];
+                            double* dst = Rb[m & 1];
+                            const int cnt = rsize(N - m);
+                            for (int x = lane; x < cnt; x += kHBlock) {
+                                const int t = kTuv.t[x], u = kTuv.u[x], v = kTuv.v[x];
+                                double val;
+                                if (x == 0)
+                                    val = Fs[m];
+                                else if (t > 0)
+                                    val = (t > 1 ? (t - 1) * src[ridx(t - 2, u, v)] : 0.0) + PQ[0] * src[ridx(t - 1, u, v)];
+                                else if (u > 0)
+                                    val = (u > 1 ? (u - 1) * src[ridx(t, u - 2, v)] : 0.0) + PQ[1] * src[ridx(t, u - 1, v)];
+                                else
+                                    val = (v > 1 ? (v - 1) * src[ridx(t, u, v - 2)] : 0.0) + PQ[2] * src[ridx(t, u, v - 1)];
+                                dst[x] = val;
+                            }
+                            __syncthreads();
+                        }
+                        const double* R = Rb[0];
+                        const double za = 2.0 * al, zb = 2.0 * be, zc = 2.0 * ga;
+                        for (int cq = lane; cq < ncomp; cq += kHBlock) {
+                            const int ia = cq / (nb * nc * nd), ib = (cq / (nc * nd)) % nb, ic = (cq / nd) % nc,
+                                      id = cq % nd;
+                            int ea[3], eb[3], ec[3], ed[3];
+                            cart_exps(A.L, ia, ea);
+                            cart_exps(B.L, ib, eb);
+                            cart_exps(C.L, ic, ec);
+                            cart_exps(D.L, id, ed);
+                            // per direction and combination: sum over t + tau = S of bra x (-1)^tau ket
+                            double cv[3][10][kHS];
+                            int len[3][10];
+                            for (int d = 0; d < 3; ++d) {
+                                const double* Ebd = Eb + d * kHE * kHE * kHT;
+                                const double* Ekd = Ek + d * kHE * kHKet * kHT;
+                                for (int cb = 0; cb < 10; ++cb) {
+                                    const int bcode = kComboBra[cb], kcode = kComboKet[cb];
+                                    const int nt = ea[d] + eb[d] + kBraOrder[bcode] + 1;
+                                    const int nk = ec[d] + ed[d] + kcode + 1;
+                                    double bv[kHT], kv[kHT];
+                                    for (int t = 0; t < nt; ++t) bv[t] = bra_op(Ebd, bcode, ea[d], eb[d], t, za, zb);
+                                    for (int t = 0; t < nk; ++t)
+                                        kv[t] = ((t & 1) ? -1.0 : 1.0) * ket_op(Ekd, kcode, ec[d], ed[d], t, zc);
+                                    len[d][cb] = nt + nk - 1;
+                                    for (int S = 0; S < nt + nk - 1; ++S) {
+                                        double s = 0.0;
+                                        for (int t = (S - nk + 1 > 0 ? S - nk + 1 : 0); t < nt && t <= S; ++t)
+                                            s += bv[t] * kv[S - t];
+                                        cv[d][cb][S] = s;
+                                    }
+                                }
+                            }
+                            const int fa = oa + ia, fb = ob + ib, fc = oc + ic, fd = od + id;
+                            const size_t n = (size_t)g.ncart;
+                            const double gC = pref * deg * 2.0 * g.Dc[fa * n + fb] * g.Dc[fc * n + fd];
+                            const double gX =
+                                -pref * deg * 0.5 * (g.Dc[fa * n + fc] * g.Dc[fb * n + fd] + g.Dc[fa * n + fd] * g.Dc[fb * n + fc]);
+                            sfor<kHPairs>([&](auto op) {
+                                constexpr PairDirs pd = pair_dirs(op);
+                                const double* cx = cv[0][pd.k[0]];
+                                const double* cy = cv[1][pd.k[1]];
+                                const double* cz = cv[2][pd.k[2]];
+                                const int lx = len[0][pd.k[0]], ly = len[1][pd.k[1]], lz = len[2][pd.k[2]];
+                                double I = 0.0;
+                                for (int S = 0; S < lx; ++S) {
+                                    double sy = 0.0;
+                                    for (int U = 0; U < ly; ++U) {
+                                        double sz = 0.0;
+                                        for (int V = 0; V < lz; ++V) sz += cz[V] * R[ridx(S, U, V)];
+                                        sy += cy[U] * sz;
+                                    }
+                                    I += cx[S] * sy;
+                                }
+                                accC[op] += gC * I;
+                                accX[op] += gX * I;
+                            });
+                        }
+                        __syncthreads();  // Ek, Fs and Rb are rewritten by the next primitive quartet
+                    }
+            }
+        // wavefront sums -> the symmetric 9 x 9 blocks over (A, B, C) x (x, y, z)
+        sfor<kHPairs>([&](auto op) {
+            constexpr PairDirs pd = pair_dirs(op);
+            const double c = wave_sum(accC[op]), x = wave_sum(accX[op]);
+            if (lane == 0) {
+                M[0][pd.x1 * 9 + pd.x2] = M[0][pd.x2 * 9 + pd.x1] = c;
+                M[1][pd.x1 * 9 + pd.x2] = M[1][pd.x2 * 9 + pd.x1] = x;
+            }
+        });
+        __syncthreads();
+        // 12 x 12 over A, B, C, D (the D rows and columns -(A + B + C)) into the atoms' entries
+        const int atoms[4] = {A.atom, B.atom, C.atom, D.atom};
+        const int n3 = 3 * g.natom;
+        for (int y = lane; y < 144; y += kHBlock) {
+            const int y1 = y / 12, y2 = y % 12;
+            double vc = 0.0, vx = 0.0;
+            for (int c1 = 0; c1 < 3; ++c1) {
+                if (y1 < 9 && c1 != y1 / 3) continue;
+                const int x1 = 3 * c1 + y1 % 3;
+                const double t1 = y1 < 9 ? 1.0 : -1.0;
+                for (int c2 = 0; c2 < 3; ++c2) {
+                    if (y2 < 9 && c2 != y2 / 3) continue;
+                    const int x2 = 3 * c2 + y2 % 3;
+                    const double t = y2 < 9 ? t1 : -t1;
+                    vc += t * M[0][x1 * 9 + x2];
+                    vx += t * M[1][x1 * 9 + x2];
+                }
+            }
+            const size_t r = (size_t)atoms[y1 / 3] * 3 + y1 % 3, col = (size_t)atoms[y2 / 3] * 3 + y2 % 3;
+            unsafeAtomicAdd(&g.HC[r * n3 + col], vc);
+            unsafeAtomicAdd(&g.HX[r * n3 + col], vx);
+        }
+        __syncthreads();  // M is rewritten by the next quartet
+    }
+}
+
+}  // namespace
+
+}  // namespace hfg
+
+extern "C" {
+
+int hfg_hessian_2e(hfg_ctx* c, const double* D, double* HC, double* HX) {
+    if (!c) return HFG_ERR_ARG;
+    if (!D || !HC || !HX) {
+        c->err = "hfg_hessian_2e: null argument";
+        return HFG_ERR_ARG;
+    }
+    hfg::OneArgs one;
+    if (int r = hfg::one_upload(c, one)) return r;
+    for (int s = 0; s < c->nshell; ++s)
+        if (c->L[s] > hfg::kHL) {
+            c->err = "hfg_hessian_2e: shells up to d";
+            return HFG_ERR_ARG;
+        }
+    // the density in the Cartesian basis: Dc = T D T^T with T the output-basis -> Cartesian map
+    const int n = c->nbf, nc = c->ncart;
+    std::vector<double> Dc((size_t)nc * nc, 0.0);
+    auto terms = [&](int s, int k, int* cidx, double* w) {  // output function k of shell s
+        if (!c->pure || c->L[s] == 0) {
+            cidx[0] = k;
+            w[0] = 1.0;
+            return 1;
+        }
+        const hfg::SphTerms t = hfg::sph_terms(c->L[s], k);
+        for (int x = 0; x < t.n; ++x) {
+            cidx[x] = t.c[x];
+            w[x] = t.w[x];
+        }
+        return t.n;
+    };
+    const std::vector<int>& out_off = c->pure ? c->pure_off : c->cart_off;
+    for (int sa = 0; sa < c->nshell; ++sa)
+        for (int sb = 0; sb < c->nshell; ++sb) {
+            const int La = c->L[sa], Lb = c->L[sb];
+            const int noa = c->pure ? 2 * La + 1 : (La + 1) * (La + 2) / 2;
+            const int nob = c->pure ? 2 * Lb + 1 : (Lb + 1) * (Lb + 2) / 2;
+            for (int p = 0; p < noa; ++p) {
+                int ca[3];
+                double wa[3];
+                const int ka = terms(sa, p, ca, wa);
+                for (int q = 0; q < nob; ++q) {
+                    int cb[3];
+                    double wb[3];
+                    const int kb = terms(sb, q, cb, wb);
+                    const double v = D[(size_t)(out_off[sa] + p) * n + out_off[sb] + q];
+                    for (int x = 0; x < ka; ++x)
+                        for (int y = 0; y < kb; ++y)
+                            Dc[(size_t)(c->cart_off[sa] + ca[x]) * nc + c->cart_off[sb] + cb[y]] += wa[x] * wb[y] * v;
+                }
+            }
+        }
+    std::vector<int2> pairs;
+    for (int a = 0; a < c->nshell; ++a)
+        for (int b = 0; b <= a; ++b) pairs.push_back(make_int2(a, b));
+    const int n3 = 3 * c->natom;
+    const size_t nh = (size_t)n3 * n3;
+    HFG_CHECK(c, c->d_one_pairs.upload(pairs, c->stream));
+    HFG_CHECK(c, c->d_hess_cart.upload(c->cart_off, c->stream));
+    HFG_CHECK(c, c->d_one_buf.ensure(Dc.size() + 2 * nh));
+    double* dDc = c->d_one_buf.p;
+    double* dH = dDc + Dc.size();
+    HFG_CHECK(c, hipMemcpyAsync(dDc, Dc.data(), Dc.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HFG_CHECK(c, hipMemsetAsync(dH, 0, 2 * nh * sizeof(double), c->stream));
+    hfg::HessArgs g{};
+    g.sh = one.sh;
+    g.alpha = one.alpha;
+    g.coef = one.coef;
+    g.xyz = one.xyz;
+    g.boys = one.boys;
+    g.pairs = c->d_one_pairs.p;
+    g.nquart = (long long)pairs.size() * ((long long)pairs.size() + 1) / 2;
+    g.cart_off = c->d_hess_cart.p;
+    g.ncart = nc;
+    g.natom = c->natom;
+    g.Dc = dDc;
+    g.prim_cut = c->opt.integral_tol / 1e4;
+    g.HC = dH;
+    g.HX = dH + nh;
+    const long long nblocks = std::min<long long>(g.nquart, 1 << 16);
+    if (nblocks > 0)
+        hipLaunchKernelGGL(hfg::hess2e_kernel, dim3((unsigned)nblocks), dim3(hfg::kHBlock), 0, c->stream, g);
+    HFG_CHECK(c, hipGetLastError());
+    HFG_CHECK(c, hipMemcpyAsync(HC, g.HC, nh * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HFG_CHECK(c, hipMemcpyAsync(HX, g.HX, nh * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HFG_CHECK(c, hipStreamSynchronize(c->stream));
+    return HFG_OK;
+}
+
+}  // extern "C"

@@ -436,7 +436,10 @@ int one_fail(hfg_ctx* c, int code, const char* msg) {
     return code;
 }
 
-// Device copies of the molecule's shells, exponents, coefficients, geometry and charges.
+}  // namespace
+
+namespace hfg {
+
 int one_upload(hfg_ctx* c, hfg::OneArgs& g) {
     if (!c->have_basis || !c->have_geom) return one_fail(c, HFG_ERR_ARG, "basis and geometry must be set first");
     if (c->batch || c->mols.size() != 1) return one_fail(c, HFG_ERR_ARG, "one-electron integrals: single-molecule contexts only");
@@ -465,14 +468,14 @@ int one_upload(hfg_ctx* c, hfg::OneArgs& g) {
     return HFG_OK;
 }
 
-}  // namespace
+}  // namespace hfg
 
 extern "C" {
 
 int hfg_oneint(hfg_ctx* c, double* S, double* T, double* V) {
     if (!c) return HFG_ERR_ARG;
     hfg::OneArgs g;
-    if (int r = one_upload(c, g)) return r;
+    if (int r = hfg::one_upload(c, g)) return r;
     std::vector<int2> pairs;
     for (int a = 0; a < c->nshell; ++a)
         for (int b = 0; b <= a; ++b) pairs.push_back(make_int2(a, b));
@@ -496,7 +499,7 @@ int hfg_oneint_gradient(hfg_ctx* c, const double* D, const double* Q, double* gr
     if (!c) return HFG_ERR_ARG;
     if (!D || !Q || !grad) return one_fail(c, HFG_ERR_ARG, "hfg_oneint_gradient: null argument");
     hfg::OneArgs g;
-    if (int r = one_upload(c, g)) return r;
+    if (int r = hfg::one_upload(c, g)) return r;
     std::vector<int2> pairs;
     for (int a = 0; a < c->nshell; ++a)
         for (int b = 0; b <= a; ++b) pairs.push_back(make_int2(a, b));

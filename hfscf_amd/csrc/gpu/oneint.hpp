@@ -2,6 +2,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+struct hfg_ctx;
+
 namespace hfg {
 
 // A shell for the one-electron kernels: angular momentum, primitive count, offset of its
@@ -27,6 +29,10 @@ struct OneArgs {
 
 // S, T and V (n x n, both triangles) of every pair in g.pairs.
 hipError_t launch_oneint(const OneArgs& g, hipStream_t s);
+// Device copies of the context's single molecule (shells, exponents, normalised coefficients,
+// geometry, charges) into g; an HFG_ error code (message in the context) if it has none.
+int one_upload(hfg_ctx* c, OneArgs& g);
+
 // grad (natom x 3, device) += 2 D d(T+V)/dX - 2 Q dS/dX over every ordered shell pair.
 hipError_t launch_oneint_grad(const OneArgs& g, const double* D, const double* Q, double* grad, hipStream_t s);
 

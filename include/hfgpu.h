@@ -200,10 +200,21 @@ int hfg_oneint_gradient(hfg_ctx* ctx, const double* D, const double* Q, double* 
  * max(x,y)(max(x,y)+1)/2 + min(x,y), count = npair (npair + 1) / 2 with npair = nbf (nbf + 1) / 2,
  * in the output basis, with the context's screening (quartets and elements hfscf's screen drops
  * stay 0; its Cartesian diagonal), as hfscf stores them for post-SCF methods and the AO -> MO
- * transform. The J/K kernels compute it (FP64) and write each element instead of digesting it. Replaces: the stored repulsion vector of hf_solver (hfscf: src/hfscf/hfscf.cpp:76,
+ * transform. The J/K kernels compute it (FP64) and write each element instead of digesting it.
+ * Replaces: the stored repulsion vector of hf_solver (hfscf: src/hfscf/hfscf.cpp:76,
  * src/basis/hfscf_trans_basis.cpp:7, src/math/hfscf_math.hpp:90; built by
  * src/integrals/hfscf_oseri.cpp:22-246). */
 int hfg_eri_vector(hfg_ctx* ctx, double* out, long long count);
+
+/* Two-electron part of the RHF Hessian of the context's single molecule at a fixed density, on
+ * the GPU: HC = sum D_ij D_kl 2 d2(ij|kl)/dX1 dX2 and HX = -sum D_ij D_kl d2(ik|jl)/dX1 dX2, each
+ * (3 natom) x (3 natom) row-major over (atom, x y z), overwritten. D: nbf x nbf (hfscf's d_mat, the
+ * output basis). Second-derivative integrals of every shell quartet (no Schwarz screen; primitive
+ * quartets below integral_tol / 1e4 skipped), contracted as they are made; shells up to d.
+ * Replaces: Mol::scf_hessian::calc_scf_hessian_tei, its m_hessianC and m_hessianEx (hfscf:
+ * src/hessian/hfscf_hessian_ints.cpp:20-313; integrals src/integrals/hfscf_oserideriv2qq.cpp:28-749,
+ * hfscf_oserideriv2pq.cpp:23-799). */
+int hfg_hessian_2e(hfg_ctx* ctx, const double* D, double* HC, double* HX);
 
 /* Autotuner: times every class launch alone over candidate launch shapes (threads per
  * workgroup x persistent workgroups) on the density of the most recent build and keeps the
