@@ -1932,6 +1932,7 @@ void hfg_destroy(hfg_ctx* c) {
     c->d_one_Z.release();
     c->d_one_buf.release();
     c->d_hess_cart.release();
+    c->d_hess_items.release();
     c->d_gc_p1.release();
     c->d_gc_K.release();
     c->d_gJ_off.release();

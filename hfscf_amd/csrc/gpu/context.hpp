@@ -225,6 +225,7 @@ struct hfg_ctx {
     DevBuf<int2> d_one_pairs;
     DevBuf<double> d_one_alpha, d_one_coef, d_one_xyz, d_one_Z, d_one_buf;
     DevBuf<int> d_hess_cart;  // first Cartesian function per shell (hfg_hessian_2e)
+    DevBuf<int4> d_hess_items;  // its work items (hfg::HItem, four ints)
     // launch graphs of the J/K build (keyed by precision and density screening)
     std::vector<hfg::JkGraph> graphs;
     bool class_timed = false;  // the last J/K build bracketed each class launch with events

@@ -22,6 +22,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstring>
+#include <utility>
 #include <vector>
 
 #include "boys.hpp"
@@ -37,12 +39,13 @@ namespace {
 constexpr int kHL = 2;                    // largest shell angular momentum
 constexpr int kHE = kHL + 3;              // exponents 0 .. L + 2 of a shifted centre
 constexpr int kHT = 2 * kHL + 5;          // Hermite index t <= i + j <= 2 L + 4
-constexpr int kHKet = kHL + 1;            // ket j (centre D, never shifted) <= L
 constexpr int kHN = 4 * kHL + 2;          // R_tuv order <= L_total + 2
 constexpr int kHR = (kHN + 1) * (kHN + 2) * (kHN + 3) / 6;
 constexpr int kHS = kHN + 1;              // convolution length of one direction
 constexpr int kHBlock = 64;               // one wavefront per shell quartet
 constexpr int kHPairs = 45;               // coordinate pairs X1 <= X2 over (A, B, C) x (x, y, z)
+// LDS for the slots' tables (one slot of the largest class: 3 x 5 x 3 x 7 + 2 x 286 + 11)
+constexpr int kHPool = 3072;
 
 // (t, u, v) of the packed R index ridx(t, u, v) = s(s+1)(s+2)/6 + (s-t)(s-t+1)/2 + v, s = t+u+v.
 struct TuvTab {
@@ -104,6 +107,13 @@ constexpr PairDirs pair_dirs(int op) {
     return r;
 }
 
+// A work item: canonical shell-pair indices P >= Q and a range [b0, b1) of the quartet's primitive
+// work (bra primitive pairs for the generic kernel, flattened primitive quartets bra-major for the
+// per-lane kernels), so that no item holds more than a few hundred primitive quartets.
+struct HItem {
+    int P, Q, b0, b1;
+};
+
 struct HessArgs {
     const OneShell* sh;
     const double* alpha;
@@ -111,7 +121,6 @@ struct HessArgs {
     const double* xyz;
     const double* boys;
     const int2* pairs;    // canonical shell pairs (a, b), a >= b
-    long long nquart;     // npairs (npairs + 1) / 2 canonical quartets
     const int* cart_off;  // first Cartesian function of each shell
     int ncart, natom;
     const double* Dc;     // ncart x ncart density in the Cartesian basis
@@ -131,12 +140,12 @@ __device__ inline void cart_exps(int L, int k, int* e) {
             }
 }
 
-// E^{ij}_t of one direction at E[(i * sj + j) * kHT + t]; zero outside 0 <= t <= i + j.
-__device__ inline double eget(const double* E, int sj, int i, int j, int t) {
-    return (i < 0 || j < 0 || t < 0 || t > i + j) ? 0.0 : E[(i * sj + j) * kHT + t];
+// E^{ij}_t of one direction at E[(i * sj + j) * st + t]; zero outside 0 <= t <= i + j.
+__device__ inline double eget(const double* E, int sj, int st, int i, int j, int t) {
+    return (i < 0 || j < 0 || t < 0 || t > i + j) ? 0.0 : E[(i * sj + j) * st + t];
 }
 
-__device__ void herm_fill(double* E, int sj, int imax, int jmax, double o2p, double xpa, double xpb) {
+__device__ void herm_fill(double* E, int sj, int st, int imax, int jmax, double o2p, double xpa, double xpb) {
     for (int i = 0; i <= imax; ++i)
         for (int j = 0; j <= jmax; ++j)
             for (int t = 0; t <= i + j; ++t) {
@@ -144,12 +153,12 @@ __device__ void herm_fill(double* E, int sj, int imax, int jmax, double o2p, dou
                 if (i == 0 && j == 0)
                     v = 1.0;
                 else if (i > 0)
-                    v = o2p * eget(E, sj, i - 1, j, t - 1) + xpa * eget(E, sj, i - 1, j, t) +
-                        (t + 1) * eget(E, sj, i - 1, j, t + 1);
+                    v = o2p * eget(E, sj, st, i - 1, j, t - 1) + xpa * eget(E, sj, st, i - 1, j, t) +
+                        (t + 1) * eget(E, sj, st, i - 1, j, t + 1);
                 else
-                    v = o2p * eget(E, sj, i, j - 1, t - 1) + xpb * eget(E, sj, i, j - 1, t) +
-                        (t + 1) * eget(E, sj, i, j - 1, t + 1);
-                E[(i * sj + j) * kHT + t] = v;
+                    v = o2p * eget(E, sj, st, i, j - 1, t - 1) + xpb * eget(E, sj, st, i, j - 1, t) +
+                        (t + 1) * eget(E, sj, st, i, j - 1, t + 1);
+                E[(i * sj + j) * st + t] = v;
             }
 }
 
@@ -157,30 +166,29 @@ __device__ void herm_fill(double* E, int sj, int imax, int jmax, double o2p, dou
 __device__ inline double bra_op(const double* E, int code, int a, int b, int t, double za, double zb) {
     constexpr int sj = kHE;
     switch (code) {
-        case 0: return eget(E, sj, a, b, t);
-        case 1: return za * eget(E, sj, a + 1, b, t) - a * eget(E, sj, a - 1, b, t);
-        case 2: return zb * eget(E, sj, a, b + 1, t) - b * eget(E, sj, a, b - 1, t);
+        case 0: return eget(E, sj, kHT, a, b, t);
+        case 1: return za * eget(E, sj, kHT, a + 1, b, t) - a * eget(E, sj, kHT, a - 1, b, t);
+        case 2: return zb * eget(E, sj, kHT, a, b + 1, t) - b * eget(E, sj, kHT, a, b - 1, t);
         case 3:
-            return za * za * eget(E, sj, a + 2, b, t) - za * (2 * a + 1) * eget(E, sj, a, b, t) +
-                   a * (a - 1) * eget(E, sj, a - 2, b, t);
+            return za * za * eget(E, sj, kHT, a + 2, b, t) - za * (2 * a + 1) * eget(E, sj, kHT, a, b, t) +
+                   a * (a - 1) * eget(E, sj, kHT, a - 2, b, t);
         case 4:
-            return za * zb * eget(E, sj, a + 1, b + 1, t) - za * b * eget(E, sj, a + 1, b - 1, t) -
-                   zb * a * eget(E, sj, a - 1, b + 1, t) + a * b * eget(E, sj, a - 1, b - 1, t);
+            return za * zb * eget(E, sj, kHT, a + 1, b + 1, t) - za * b * eget(E, sj, kHT, a + 1, b - 1, t) -
+                   zb * a * eget(E, sj, kHT, a - 1, b + 1, t) + a * b * eget(E, sj, kHT, a - 1, b - 1, t);
         default:
-            return zb * zb * eget(E, sj, a, b + 2, t) - zb * (2 * b + 1) * eget(E, sj, a, b, t) +
-                   b * (b - 1) * eget(E, sj, a, b - 2, t);
+            return zb * zb * eget(E, sj, kHT, a, b + 2, t) - zb * (2 * b + 1) * eget(E, sj, kHT, a, b, t) +
+                   b * (b - 1) * eget(E, sj, kHT, a, b - 2, t);
     }
 }
 
-// Shifted ket coefficient (zc = 2 gamma).
-__device__ inline double ket_op(const double* E, int code, int c, int d, int t, double zc) {
-    constexpr int sj = kHKet;
+// Shifted ket coefficient (zc = 2 gamma; the ket table's strides sj, st).
+__device__ inline double ket_op(const double* E, int sj, int st, int code, int c, int d, int t, double zc) {
     switch (code) {
-        case 0: return eget(E, sj, c, d, t);
-        case 1: return zc * eget(E, sj, c + 1, d, t) - c * eget(E, sj, c - 1, d, t);
+        case 0: return eget(E, sj, st, c, d, t);
+        case 1: return zc * eget(E, sj, st, c + 1, d, t) - c * eget(E, sj, st, c - 1, d, t);
         default:
-            return zc * zc * eget(E, sj, c + 2, d, t) - zc * (2 * c + 1) * eget(E, sj, c, d, t) +
-                   c * (c - 1) * eget(E, sj, c - 2, d, t);
+            return zc * zc * eget(E, sj, st, c + 2, d, t) - zc * (2 * c + 1) * eget(E, sj, st, c, d, t) +
+                   c * (c - 1) * eget(E, sj, st, c - 2, d, t);
     }
 }
 
@@ -198,25 +206,46 @@ __device__ inline void boys_upto(int N, double T, const double* tab, double* F) 
     }
 }
 
+__device__ inline double T_of(double aR, const double* PQ) { return aR * (PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]); }
+
 __device__ inline double wave_sum(double v) {
     for (int o = kHBlock / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kHBlock);
     return v;
 }
 
-__global__ __launch_bounds__(kHBlock) void hess2e_kernel(HessArgs g) {
+// Atom entries of HC and HX from the item's symmetric 9 x 9 blocks M over (A, B, C) x (x, y, z):
+// the 12 x 12 block of the four centres (the D rows and columns -(A + B + C)) added with atomics.
+__device__ void finish_item(const HessArgs& g, const int atoms[4], double (*M)[81]) {
+    const int n3 = 3 * g.natom;
+    for (int y = threadIdx.x; y < 144; y += kHBlock) {
+        const int y1 = y / 12, y2 = y % 12;
+        double vc = 0.0, vx = 0.0;
+        for (int c1 = 0; c1 < 3; ++c1) {
+            if (y1 < 9 && c1 != y1 / 3) continue;
+            const int x1 = 3 * c1 + y1 % 3;
+            const double t1 = y1 < 9 ? 1.0 : -1.0;
+            for (int c2 = 0; c2 < 3; ++c2) {
+                if (y2 < 9 && c2 != y2 / 3) continue;
+                const int x2 = 3 * c2 + y2 % 3;
+                const double t = y2 < 9 ? t1 : -t1;
+                vc += t * M[0][x1 * 9 + x2];
+                vx += t * M[1][x1 * 9 + x2];
+            }
+        }
+        const size_t r = (size_t)atoms[y1 / 3] * 3 + y1 % 3, col = (size_t)atoms[y2 / 3] * 3 + y2 % 3;
+        unsafeAtomicAdd(&g.HC[r * n3 + col], vc);
+        unsafeAtomicAdd(&g.HX[r * n3 + col], vx);
+    }
+}
+
+__global__ __launch_bounds__(kHBlock) void hess2e_kernel(HessArgs g, const HItem* items, int nitems) {
     __shared__ double Eb[3 * kHE * kHE * kHT];
-    __shared__ double Ek[3 * kHE * kHKet * kHT];
-    __shared__ double Rb[2][kHR];
-    __shared__ double Fs[kHN + 1];
+    __shared__ double pool[kHPool];  // per slot: ket E^{ij}_t (3 directions), R ping-pong, scaled F_m
     __shared__ double M[2][81];
     const int lane = threadIdx.x;
-    for (long long q = blockIdx.x; q < g.nquart; q += gridDim.x) {
-        // quartet q -> canonical pair indices P >= Q
-        long long P = (long long)((sqrt(8.0 * (double)q + 1.0) - 1.0) * 0.5);
-        while (P * (P + 1) / 2 > q) --P;
-        while ((P + 1) * (P + 2) / 2 <= q) ++P;
-        const long long Qp = q - P * (P + 1) / 2;
-        const int2 pab = g.pairs[P], pcd = g.pairs[Qp];
+    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+        const HItem w = items[it];
+        const int2 pab = g.pairs[w.P], pcd = g.pairs[w.Q];
         const int s4[4] = {pab.x, pab.y, pcd.x, pcd.y};
         const OneShell A = g.sh[s4[0]], B = g.sh[s4[1]], C = g.sh[s4[2]], D = g.sh[s4[3]];
         const double* ra = g.xyz + 3 * A.atom;
@@ -226,62 +255,96 @@ __global__ __launch_bounds__(kHBlock) void hess2e_kernel(HessArgs g) {
         double deg = 8.0;
         if (pab.x == pab.y) deg *= 0.5;
         if (pcd.x == pcd.y) deg *= 0.5;
-        if (P == Qp) deg *= 0.5;
+        if (w.P == w.Q) deg *= 0.5;
         const int na = (A.L + 1) * (A.L + 2) / 2, nb = (B.L + 1) * (B.L + 2) / 2;
         const int nc = (C.L + 1) * (C.L + 2) / 2, nd = (D.L + 1) * (D.L + 2) / 2;
         const int ncomp = na * nb * nc * nd;
         const int N = A.L + B.L + C.L + D.L + 2;
+        const int RN = rsize(N);
         const int oa = g.cart_off[s4[0]], ob = g.cart_off[s4[1]], oc = g.cart_off[s4[2]], od = g.cart_off[s4[3]];
         double ab2 = 0.0, cd2 = 0.0;
         for (int d = 0; d < 3; ++d) {
             ab2 += (ra[d] - rb[d]) * (ra[d] - rb[d]);
             cd2 += (rc[d] - rd[d]) * (rc[d] - rd[d]);
         }
-        double accC[kHPairs], accX[kHPairs];
-        for (int k = 0; k < kHPairs; ++k) accC[k] = accX[k] = 0.0;
-        for (int i = 0; i < A.np; ++i)
-            for (int j = 0; j < B.np; ++j) {
+        // slots: groups of ncpad lanes, each on its own ket primitive pair with its own tables
+        const int ncpad = ncomp > 32 ? 64 : ncomp > 16 ? 32 : ncomp > 8 ? 16 : ncomp > 4 ? 8 : ncomp > 2 ? 4 : ncomp;
+        const int ksj = D.L + 1, kst = C.L + D.L + 3, kdir = (C.L + 3) * ksj * kst;
+        const int slot_size = 3 * kdir + 2 * RN + N + 1;
+        const int nslot = min(kHBlock / ncpad, kHPool / slot_size);
+        const int slot = lane / ncpad, sl = lane - slot * ncpad;
+        double* Ek = pool + (slot < nslot ? slot : 0) * slot_size;
+        double* Rb0 = Ek + 3 * kdir;
+        double* Rb1 = Rb0 + RN;
+        double* Fs = Rb1 + RN;
+        const int nket = C.np * D.np;
+        for (int k = lane; k < 162; k += kHBlock) M[k / 81][k % 81] = 0.0;
+        // chunks of 64 component quartets (one unless the class has more), each over every
+        // primitive quartet of the item, with one component quartet per lane
+        for (int cbase = 0; cbase < ncomp; cbase += kHBlock) {
+            const int cq = cbase + sl;
+            const bool has_comp = cq < ncomp;
+            const int ia = cq / (nb * nc * nd), ib = (cq / (nc * nd)) % nb, ic = (cq / nd) % nc, id = cq % nd;
+            int ea[3] = {0, 0, 0}, eb[3] = {0, 0, 0}, ec[3] = {0, 0, 0}, ed[3] = {0, 0, 0};
+            if (has_comp) {
+                cart_exps(A.L, ia, ea);
+                cart_exps(B.L, ib, eb);
+                cart_exps(C.L, ic, ec);
+                cart_exps(D.L, id, ed);
+            }
+            double acc[kHPairs];
+            sfor<kHPairs>([&](auto op) { acc[op] = 0.0; });
+            for (int bp = w.b0; bp < w.b1; ++bp) {
+                const int i = bp / B.np, j = bp % B.np;
                 const double al = g.alpha[A.po + i], be = g.alpha[B.po + j], p = al + be;
                 const double Kab = exp(-al * be / p * ab2) * g.coef[A.po + i] * g.coef[B.po + j];
                 double Pc[3];
                 for (int d = 0; d < 3; ++d) Pc[d] = (al * ra[d] + be * rb[d]) / p;
-                // the previous quartet's last use of Eb ended at a barrier
+                // the previous use of Eb ended at a barrier
                 if (lane < 3)
-                    herm_fill(Eb + lane * kHE * kHE * kHT, kHE, A.L + 2, B.L + 2, 0.5 / p, Pc[lane] - ra[lane],
+                    herm_fill(Eb + lane * kHE * kHE * kHT, kHE, kHT, A.L + 2, B.L + 2, 0.5 / p, Pc[lane] - ra[lane],
                               Pc[lane] - rb[lane]);
-                for (int k = 0; k < C.np; ++k)
-                    for (int l = 0; l < D.np; ++l) {
-                        const double ga = g.alpha[C.po + k], de = g.alpha[D.po + l], qx = ga + de;
+                for (int kbase = 0; kbase < nket; kbase += nslot) {
+                    const int kp = kbase + slot;
+                    bool active = slot < nslot && kp < nket;
+                    double ga = 1.0, pref = 0.0, aR = 1.0, PQ[3] = {0.0, 0.0, 0.0};
+                    if (active) {
+                        const int k = kp / D.np, l = kp % D.np;
+                        ga = g.alpha[C.po + k];
+                        const double de = g.alpha[D.po + l], qx = ga + de;
                         const double Kcd = exp(-ga * de / qx * cd2) * g.coef[C.po + k] * g.coef[D.po + l];
-                        const double pref = 34.986836655249725693 * Kab * Kcd / (p * qx * sqrt(p + qx));  // 2 pi^2.5
-                        if (fabs(pref) < g.prim_cut) continue;  // uniform over the wavefront
-                        double Qc[3], PQ[3];
+                        pref = 34.986836655249725693 * Kab * Kcd / (p * qx * sqrt(p + qx));  // 2 pi^2.5
+                        active = fabs(pref) >= g.prim_cut;
+                        double Qc[3];
                         for (int d = 0; d < 3; ++d) {
                             Qc[d] = (ga * rc[d] + de * rd[d]) / qx;
                             PQ[d] = Pc[d] - Qc[d];
                         }
-                        const double aR = p * qx / (p + qx);
-                        if (lane < 3)
-                            herm_fill(Ek + lane * kHE * kHKet * kHT, kHKet, C.L + 2, D.L, 0.5 / qx, Qc[lane] - rc[lane],
-                                      Qc[lane] - rd[lane]);
-                        if (lane == 3) {
-                            double F[kHN + 1];
-                            boys_upto(N, aR * (PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]), g.boys, F);
-                            double f = 1.0;
-                            for (int m = 0; m <= N; ++m) {
-                                Fs[m] = f * F[m];
-                                f *= -2.0 * aR;
+                        aR = p * qx / (p + qx);
+                        if (active) {
+                            for (int d = sl; d < 3; d += ncpad)
+                                herm_fill(Ek + d * kdir, ksj, kst, C.L + 2, D.L, 0.5 / qx, Qc[d] - rc[d], Qc[d] - rd[d]);
+                            if (sl == 0) {
+                                double F[kHN + 1];
+                                boys_upto(N, T_of(aR, PQ), g.boys, F);
+                                double f = 1.0;
+                                for (int m = 0; m <= N; ++m) {
+                                    Fs[m] = f * F[m];
+                                    f *= -2.0 * aR;
+                                }
                             }
                         }
-                        __syncthreads();
-                        // R^(m)_tuv from order N down to 0; level m lives in Rb[m & 1]
-                        if (lane == 0) Rb[N & 1][0] = Fs[N];
-                        __syncthreads();
-                        for (int m = N - 1; m >= 0; --m) {
-                            const double* src = Rb[(m + 1) & 1];
-                            double* dst = Rb[m & 1];
+                    }
+                    __syncthreads();
+                    // R^(m)_tuv from order N down to 0 (level m in Rb0 for even m, Rb1 for odd m)
+                    if (active && sl == 0) ((N & 1) ? Rb1 : Rb0)[0] = Fs[N];
+                    __syncthreads();
+                    for (int m = N - 1; m >= 0; --m) {
+                        if (active) {
+                            const double* src = ((m + 1) & 1) ? Rb1 : Rb0;
+                            double* dst = (m & 1) ? Rb1 : Rb0;
                             const int cnt = rsize(N - m);
-                            for (int x = lane; x < cnt; x += kHBlock) {
+                            for (int x = sl; x < cnt; x += ncpad) {
                                 const int t = kTuv.t[x], u = kTuv.u[x], v = kTuv.v[x];
                                 double val;
                                 if (x == 0)
@@ -294,102 +357,82 @@ __global__ __launch_bounds__(kHBlock) void hess2e_kernel(HessArgs g) {
                                     val = (v > 1 ? (v - 1) * src[ridx(t, u, v - 2)] : 0.0) + PQ[2] * src[ridx(t, u, v - 1)];
                                 dst[x] = val;
                             }
-                            __syncthreads();
                         }
-                        const double* R = Rb[0];
+                        __syncthreads();
+                    }
+                    if (active && has_comp) {
+                        const double* R = Rb0;
                         const double za = 2.0 * al, zb = 2.0 * be, zc = 2.0 * ga;
-                        for (int cq = lane; cq < ncomp; cq += kHBlock) {
-                            const int ia = cq / (nb * nc * nd), ib = (cq / (nc * nd)) % nb, ic = (cq / nd) % nc,
-                                      id = cq % nd;
-                            int ea[3], eb[3], ec[3], ed[3];
-                            cart_exps(A.L, ia, ea);
-                            cart_exps(B.L, ib, eb);
-                            cart_exps(C.L, ic, ec);
-                            cart_exps(D.L, id, ed);
-                            // per direction and combination: sum over t + tau = S of bra x (-1)^tau ket
-                            double cv[3][10][kHS];
-                            int len[3][10];
-                            for (int d = 0; d < 3; ++d) {
-                                const double* Ebd = Eb + d * kHE * kHE * kHT;
-                                const double* Ekd = Ek + d * kHE * kHKet * kHT;
-                                for (int cb = 0; cb < 10; ++cb) {
-                                    const int bcode = kComboBra[cb], kcode = kComboKet[cb];
-                                    const int nt = ea[d] + eb[d] + kBraOrder[bcode] + 1;
-                                    const int nk = ec[d] + ed[d] + kcode + 1;
-                                    double bv[kHT], kv[kHT];
-                                    for (int t = 0; t < nt; ++t) bv[t] = bra_op(Ebd, bcode, ea[d], eb[d], t, za, zb);
-                                    for (int t = 0; t < nk; ++t)
-                                        kv[t] = ((t & 1) ? -1.0 : 1.0) * ket_op(Ekd, kcode, ec[d], ed[d], t, zc);
-                                    len[d][cb] = nt + nk - 1;
-                                    for (int S = 0; S < nt + nk - 1; ++S) {
-                                        double s = 0.0;
-                                        for (int t = (S - nk + 1 > 0 ? S - nk + 1 : 0); t < nt && t <= S; ++t)
-                                            s += bv[t] * kv[S - t];
-                                        cv[d][cb][S] = s;
-                                    }
+                        // per direction and combination: sum over t + tau = S of bra x (-1)^tau ket
+                        double cv[3][10][kHS];
+                        int len[3][10];
+                        for (int d = 0; d < 3; ++d) {
+                            const double* Ebd = Eb + d * kHE * kHE * kHT;
+                            const double* Ekd = Ek + d * kdir;
+                            for (int cb = 0; cb < 10; ++cb) {
+                                const int bcode = kComboBra[cb], kcode = kComboKet[cb];
+                                const int nt = ea[d] + eb[d] + kBraOrder[bcode] + 1;
+                                const int nk = ec[d] + ed[d] + kcode + 1;
+                                double bv[kHT], kv[kHT];
+                                for (int t = 0; t < nt; ++t) bv[t] = bra_op(Ebd, bcode, ea[d], eb[d], t, za, zb);
+                                for (int t = 0; t < nk; ++t)
+                                    kv[t] = ((t & 1) ? -1.0 : 1.0) * ket_op(Ekd, ksj, kst, kcode, ec[d], ed[d], t, zc);
+                                len[d][cb] = nt + nk - 1;
+                                for (int S = 0; S < nt + nk - 1; ++S) {
+                                    double sacc = 0.0;
+                                    for (int t = (S - nk + 1 > 0 ? S - nk + 1 : 0); t < nt && t <= S; ++t)
+                                        sacc += bv[t] * kv[S - t];
+                                    cv[d][cb][S] = sacc;
                                 }
                             }
-                            const int fa = oa + ia, fb = ob + ib, fc = oc + ic, fd = od + id;
-                            const size_t n = (size_t)g.ncart;
-                            const double gC = pref * deg * 2.0 * g.Dc[fa * n + fb] * g.Dc[fc * n + fd];
-                            const double gX =
-                                -pref * deg * 0.5 * (g.Dc[fa * n + fc] * g.Dc[fb * n + fd] + g.Dc[fa * n + fd] * g.Dc[fb * n + fc]);
-                            sfor<kHPairs>([&](auto op) {
-                                constexpr PairDirs pd = pair_dirs(op);
-                                const double* cx = cv[0][pd.k[0]];
-                                const double* cy = cv[1][pd.k[1]];
-                                const double* cz = cv[2][pd.k[2]];
-                                const int lx = len[0][pd.k[0]], ly = len[1][pd.k[1]], lz = len[2][pd.k[2]];
-                                double I = 0.0;
-                                for (int S = 0; S < lx; ++S) {
-                                    double sy = 0.0;
-                                    for (int U = 0; U < ly; ++U) {
-                                        double sz = 0.0;
-                                        for (int V = 0; V < lz; ++V) sz += cz[V] * R[ridx(S, U, V)];
-                                        sy += cy[U] * sz;
-                                    }
-                                    I += cx[S] * sy;
-                                }
-                                accC[op] += gC * I;
-                                accX[op] += gX * I;
-                            });
                         }
-                        __syncthreads();  // Ek, Fs and Rb are rewritten by the next primitive quartet
+                        sfor<kHPairs>([&](auto op) {
+                            constexpr PairDirs pd = pair_dirs(op);
+                            const double* cx = cv[0][pd.k[0]];
+                            const double* cy = cv[1][pd.k[1]];
+                            const double* cz = cv[2][pd.k[2]];
+                            const int lx = len[0][pd.k[0]], ly = len[1][pd.k[1]], lz = len[2][pd.k[2]];
+                            double I = 0.0;
+                            for (int S = 0; S < lx; ++S) {
+                                double sy = 0.0;
+                                for (int U = 0; U < ly; ++U) {
+                                    double sz = 0.0;
+                                    for (int V = 0; V < lz; ++V) sz += cz[V] * R[ridx(S, U, V)];
+                                    sy += cy[U] * sz;
+                                }
+                                I += cx[S] * sy;
+                            }
+                            acc[op] += pref * I;
+                        });
                     }
-            }
-        // wavefront sums -> the symmetric 9 x 9 blocks over (A, B, C) x (x, y, z)
-        sfor<kHPairs>([&](auto op) {
-            constexpr PairDirs pd = pair_dirs(op);
-            const double c = wave_sum(accC[op]), x = wave_sum(accX[op]);
-            if (lane == 0) {
-                M[0][pd.x1 * 9 + pd.x2] = M[0][pd.x2 * 9 + pd.x1] = c;
-                M[1][pd.x1 * 9 + pd.x2] = M[1][pd.x2 * 9 + pd.x1] = x;
-            }
-        });
-        __syncthreads();
-        // 12 x 12 over A, B, C, D (the D rows and columns -(A + B + C)) into the atoms' entries
-        const int atoms[4] = {A.atom, B.atom, C.atom, D.atom};
-        const int n3 = 3 * g.natom;
-        for (int y = lane; y < 144; y += kHBlock) {
-            const int y1 = y / 12, y2 = y % 12;
-            double vc = 0.0, vx = 0.0;
-            for (int c1 = 0; c1 < 3; ++c1) {
-                if (y1 < 9 && c1 != y1 / 3) continue;
-                const int x1 = 3 * c1 + y1 % 3;
-                const double t1 = y1 < 9 ? 1.0 : -1.0;
-                for (int c2 = 0; c2 < 3; ++c2) {
-                    if (y2 < 9 && c2 != y2 / 3) continue;
-                    const int x2 = 3 * c2 + y2 % 3;
-                    const double t = y2 < 9 ? t1 : -t1;
-                    vc += t * M[0][x1 * 9 + x2];
-                    vx += t * M[1][x1 * 9 + x2];
+                    __syncthreads();  // the slots' tables are rewritten by the next round
                 }
             }
-            const size_t r = (size_t)atoms[y1 / 3] * 3 + y1 % 3, col = (size_t)atoms[y2 / 3] * 3 + y2 % 3;
-            unsafeAtomicAdd(&g.HC[r * n3 + col], vc);
-            unsafeAtomicAdd(&g.HX[r * n3 + col], vx);
+            // densities of the lane's component quartet, the wavefront's sums into M
+            double gC = 0.0, gX = 0.0;
+            if (has_comp) {
+                const int fa = oa + ia, fb = ob + ib, fc = oc + ic, fd = od + id;
+                const size_t n = (size_t)g.ncart;
+                gC = deg * 2.0 * g.Dc[fa * n + fb] * g.Dc[fc * n + fd];
+                gX = -deg * 0.5 * (g.Dc[fa * n + fc] * g.Dc[fb * n + fd] + g.Dc[fa * n + fd] * g.Dc[fb * n + fc]);
+            }
+            sfor<kHPairs>([&](auto op) {
+                constexpr PairDirs pd = pair_dirs(op);
+                const double c = wave_sum(gC * acc[op]), x = wave_sum(gX * acc[op]);
+                if (lane == 0) {
+                    M[0][pd.x1 * 9 + pd.x2] += c;
+                    M[1][pd.x1 * 9 + pd.x2] += x;
+                    if (pd.x1 != pd.x2) {
+                        M[0][pd.x2 * 9 + pd.x1] += c;
+                        M[1][pd.x2 * 9 + pd.x1] += x;
+                    }
+                }
+            });
         }
-        __syncthreads();  // M is rewritten by the next quartet
+        __syncthreads();
+        const int atoms[4] = {A.atom, B.atom, C.atom, D.atom};
+        finish_item(g, atoms, M);
+        __syncthreads();  // M is rewritten by the next item
     }
 }
 
@@ -468,7 +511,6 @@ int hfg_hessian_2e(hfg_ctx* c, const double* D, double* HC, double* HX) {
     g.xyz = one.xyz;
     g.boys = one.boys;
     g.pairs = c->d_one_pairs.p;
-    g.nquart = (long long)pairs.size() * ((long long)pairs.size() + 1) / 2;
     g.cart_off = c->d_hess_cart.p;
     g.ncart = nc;
     g.natom = c->natom;
@@ -476,9 +518,32 @@ int hfg_hessian_2e(hfg_ctx* c, const double* D, double* HC, double* HX) {
     g.prim_cut = c->opt.integral_tol / 1e4;
     g.HC = dH;
     g.HX = dH + nh;
-    const long long nblocks = std::min<long long>(g.nquart, 1 << 16);
-    if (nblocks > 0)
-        hipLaunchKernelGGL(hfg::hess2e_kernel, dim3((unsigned)nblocks), dim3(hfg::kHBlock), 0, c->stream, g);
+    // work items of at most kItemPrims primitive quartets, in whole bra primitive pairs, grouped by
+    // the order N = la + lb + lc + ld + 2 (neighbouring workgroups then take items of similar cost)
+    constexpr int kItemPrims = 512;
+    std::vector<std::vector<hfg::HItem>> by_n(hfg::kHN + 1);
+    const int np = (int)pairs.size();
+    for (int P = 0; P < np; ++P)
+        for (int Q = 0; Q <= P; ++Q) {
+            const int a = pairs[P].x, b = pairs[P].y, cc = pairs[Q].x, d = pairs[Q].y;
+            const int N = c->L[a] + c->L[b] + c->L[cc] + c->L[d] + 2;
+            const int nbra = c->nprim[a] * c->nprim[b], nket = c->nprim[cc] * c->nprim[d];
+            const int per = std::max(1, kItemPrims / nket);
+            for (int b0 = 0; b0 < nbra; b0 += per) by_n[N].push_back(hfg::HItem{P, Q, b0, std::min(nbra, b0 + per)});
+        }
+    std::vector<int4> packed;
+    std::vector<size_t> first(hfg::kHN + 2, 0);
+    for (int N = 0; N <= hfg::kHN; ++N) {
+        first[N] = packed.size();
+        for (const hfg::HItem& it : by_n[N]) packed.push_back(make_int4(it.P, it.Q, it.b0, it.b1));
+    }
+    first[hfg::kHN + 1] = packed.size();
+    HFG_CHECK(c, c->d_hess_items.upload(packed, c->stream));
+    const hfg::HItem* d_items = reinterpret_cast<const hfg::HItem*>(c->d_hess_items.p);
+    static_assert(sizeof(hfg::HItem) == sizeof(int4), "work items travel as int4");
+    if (!packed.empty())
+        hipLaunchKernelGGL(hfg::hess2e_kernel, dim3((unsigned)std::min<size_t>(packed.size(), 1 << 16)), dim3(hfg::kHBlock),
+                           0, c->stream, g, d_items, (int)packed.size());
     HFG_CHECK(c, hipGetLastError());
     HFG_CHECK(c, hipMemcpyAsync(HC, g.HC, nh * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HFG_CHECK(c, hipMemcpyAsync(HX, g.HX, nh * sizeof(double), hipMemcpyDeviceToHost, c->stream));
