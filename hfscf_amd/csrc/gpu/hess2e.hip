@@ -44,8 +44,8 @@ constexpr int kHR = (kHN + 1) * (kHN + 2) * (kHN + 3) / 6;
 constexpr int kHS = kHN + 1;              // convolution length of one direction
 constexpr int kHBlock = 64;               // one wavefront per shell quartet
 constexpr int kHPairs = 45;               // coordinate pairs X1 <= X2 over (A, B, C) x (x, y, z)
-// LDS for the slots' tables (one slot of the largest class: 3 x 5 x 3 x 7 + 2 x 286 + 11)
-constexpr int kHPool = 3072;
+// LDS for the slots' tables (one slot of the largest class: 3 x 5 x 3 x 7 + 2 x 286 + 11 = 898)
+constexpr int kHPool = 1536;
 
 // (t, u, v) of the packed R index ridx(t, u, v) = s(s+1)(s+2)/6 + (s-t)(s-t+1)/2 + v, s = t+u+v.
 struct TuvTab {
@@ -107,6 +107,42 @@ constexpr PairDirs pair_dirs(int op) {
     return r;
 }
 
+// pair_dirs of every coordinate pair, for loops over the pairs that are not unrolled.
+struct PairTab {
+    unsigned char k[kHPairs][3];
+    unsigned char x1[kHPairs], x2[kHPairs];
+};
+constexpr PairTab make_pairs() {
+    PairTab t{};
+    for (int op = 0; op < kHPairs; ++op) {
+        const PairDirs pd = pair_dirs(op);
+        for (int d = 0; d < 3; ++d) t.k[op][d] = (unsigned char)pd.k[d];
+        t.x1[op] = (unsigned char)pd.x1;
+        t.x2[op] = (unsigned char)pd.x2;
+    }
+    return t;
+}
+__constant__ PairTab kPairs = make_pairs();
+
+// How a wavefront lays out a class: ncpad lanes per slot (the component quartets rounded up to a
+// power of two, 64 above 32), nslot slots, each with its own ket tables (ksj, kst: strides of the
+// ket E^{ij}_t; kdir doubles per direction) and R tables (rn entries).
+struct SlotShape {
+    int ncomp, ncpad, nslot, ksj, kst, kdir, rn, slot_size;
+};
+__host__ __device__ inline SlotShape slot_shape(int la, int lb, int lc, int ld) {
+    SlotShape s;
+    s.ncomp = (la + 1) * (la + 2) / 2 * ((lb + 1) * (lb + 2) / 2) * ((lc + 1) * (lc + 2) / 2) * ((ld + 1) * (ld + 2) / 2);
+    s.ncpad = s.ncomp > 32 ? 64 : s.ncomp > 16 ? 32 : s.ncomp > 8 ? 16 : s.ncomp > 4 ? 8 : s.ncomp > 2 ? 4 : s.ncomp;
+    s.ksj = ld + 1;
+    s.kst = lc + ld + 3;
+    s.kdir = (lc + 3) * s.ksj * s.kst;
+    s.rn = rsize(la + lb + lc + ld + 2);
+    s.slot_size = 3 * s.kdir + 2 * s.rn + la + lb + lc + ld + 3;
+    s.nslot = kHBlock / s.ncpad < kHPool / s.slot_size ? kHBlock / s.ncpad : kHPool / s.slot_size;
+    return s;
+}
+
 // A work item: canonical shell-pair indices P >= Q and a range [b0, b1) of the quartet's primitive
 // work (bra primitive pairs for the generic kernel, flattened primitive quartets bra-major for the
 // per-lane kernels), so that no item holds more than a few hundred primitive quartets.
@@ -129,7 +165,7 @@ struct HessArgs {
     double* HX;           // (3 natom)^2 exchange part
 };
 
-__device__ inline void cart_exps(int L, int k, int* e) {
+HFG_INLINE void cart_exps(int L, int k, int* e) {
     int n = 0;
     for (int i = 0; i <= L; ++i)
         for (int j = 0; j <= i; ++j, ++n)
@@ -141,11 +177,11 @@ __device__ inline void cart_exps(int L, int k, int* e) {
 }
 
 // E^{ij}_t of one direction at E[(i * sj + j) * st + t]; zero outside 0 <= t <= i + j.
-__device__ inline double eget(const double* E, int sj, int st, int i, int j, int t) {
+HFG_INLINE double eget(const double* E, int sj, int st, int i, int j, int t) {
     return (i < 0 || j < 0 || t < 0 || t > i + j) ? 0.0 : E[(i * sj + j) * st + t];
 }
 
-__device__ void herm_fill(double* E, int sj, int st, int imax, int jmax, double o2p, double xpa, double xpb) {
+HFG_INLINE void herm_fill(double* E, int sj, int st, int imax, int jmax, double o2p, double xpa, double xpb) {
     for (int i = 0; i <= imax; ++i)
         for (int j = 0; j <= jmax; ++j)
             for (int t = 0; t <= i + j; ++t) {
@@ -163,7 +199,7 @@ __device__ void herm_fill(double* E, int sj, int st, int imax, int jmax, double 
 }
 
 // Shifted bra coefficient of code `code` (za = 2 alpha, zb = 2 beta).
-__device__ inline double bra_op(const double* E, int code, int a, int b, int t, double za, double zb) {
+HFG_INLINE double bra_op(const double* E, int code, int a, int b, int t, double za, double zb) {
     constexpr int sj = kHE;
     switch (code) {
         case 0: return eget(E, sj, kHT, a, b, t);
@@ -182,7 +218,7 @@ __device__ inline double bra_op(const double* E, int code, int a, int b, int t, 
 }
 
 // Shifted ket coefficient (zc = 2 gamma; the ket table's strides sj, st).
-__device__ inline double ket_op(const double* E, int sj, int st, int code, int c, int d, int t, double zc) {
+HFG_INLINE double ket_op(const double* E, int sj, int st, int code, int c, int d, int t, double zc) {
     switch (code) {
         case 0: return eget(E, sj, st, c, d, t);
         case 1: return zc * eget(E, sj, st, c + 1, d, t) - c * eget(E, sj, st, c - 1, d, t);
@@ -192,7 +228,7 @@ __device__ inline double ket_op(const double* E, int sj, int st, int code, int c
     }
 }
 
-__device__ inline void boys_upto(int N, double T, const double* tab, double* F) {
+HFG_INLINE void boys_upto(int N, double T, const double* tab, double* F) {
     switch (N) {
         case 2: boys_eval<2, double>(T, tab, 1.0, F); break;
         case 3: boys_eval<3, double>(T, tab, 1.0, F); break;
@@ -206,16 +242,16 @@ __device__ inline void boys_upto(int N, double T, const double* tab, double* F) 
     }
 }
 
-__device__ inline double T_of(double aR, const double* PQ) { return aR * (PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]); }
+HFG_INLINE double T_of(double aR, const double* PQ) { return aR * (PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]); }
 
-__device__ inline double wave_sum(double v) {
+HFG_INLINE double wave_sum(double v) {
     for (int o = kHBlock / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kHBlock);
     return v;
 }
 
 // Atom entries of HC and HX from the item's symmetric 9 x 9 blocks M over (A, B, C) x (x, y, z):
 // the 12 x 12 block of the four centres (the D rows and columns -(A + B + C)) added with atomics.
-__device__ void finish_item(const HessArgs& g, const int atoms[4], double (*M)[81]) {
+HFG_INLINE void finish_item(const HessArgs& g, const int atoms[4], double (*M)[81]) {
     const int n3 = 3 * g.natom;
     for (int y = threadIdx.x; y < 144; y += kHBlock) {
         const int y1 = y / 12, y2 = y % 12;
@@ -238,7 +274,8 @@ __device__ void finish_item(const HessArgs& g, const int atoms[4], double (*M)[8
     }
 }
 
-__global__ __launch_bounds__(kHBlock) void hess2e_kernel(HessArgs g, const HItem* items, int nitems) {
+__global__ __launch_bounds__(kHBlock) __attribute__((amdgpu_num_vgpr(128))) void hess2e_kernel(HessArgs g, const HItem* items,
+                                                                                          int nitems) {
     __shared__ double Eb[3 * kHE * kHE * kHT];
     __shared__ double pool[kHPool];  // per slot: ket E^{ij}_t (3 directions), R ping-pong, scaled F_m
     __shared__ double M[2][81];
@@ -256,11 +293,12 @@ __global__ __launch_bounds__(kHBlock) void hess2e_kernel(HessArgs g, const HItem
         if (pab.x == pab.y) deg *= 0.5;
         if (pcd.x == pcd.y) deg *= 0.5;
         if (w.P == w.Q) deg *= 0.5;
-        const int na = (A.L + 1) * (A.L + 2) / 2, nb = (B.L + 1) * (B.L + 2) / 2;
+        const int nb = (B.L + 1) * (B.L + 2) / 2;
         const int nc = (C.L + 1) * (C.L + 2) / 2, nd = (D.L + 1) * (D.L + 2) / 2;
-        const int ncomp = na * nb * nc * nd;
+        const SlotShape shp = slot_shape(A.L, B.L, C.L, D.L);
+        const int ncomp = shp.ncomp;
         const int N = A.L + B.L + C.L + D.L + 2;
-        const int RN = rsize(N);
+        const int RN = shp.rn;
         const int oa = g.cart_off[s4[0]], ob = g.cart_off[s4[1]], oc = g.cart_off[s4[2]], od = g.cart_off[s4[3]];
         double ab2 = 0.0, cd2 = 0.0;
         for (int d = 0; d < 3; ++d) {
@@ -268,10 +306,8 @@ __global__ __launch_bounds__(kHBlock) void hess2e_kernel(HessArgs g, const HItem
             cd2 += (rc[d] - rd[d]) * (rc[d] - rd[d]);
         }
         // slots: groups of ncpad lanes, each on its own ket primitive pair with its own tables
-        const int ncpad = ncomp > 32 ? 64 : ncomp > 16 ? 32 : ncomp > 8 ? 16 : ncomp > 4 ? 8 : ncomp > 2 ? 4 : ncomp;
-        const int ksj = D.L + 1, kst = C.L + D.L + 3, kdir = (C.L + 3) * ksj * kst;
-        const int slot_size = 3 * kdir + 2 * RN + N + 1;
-        const int nslot = min(kHBlock / ncpad, kHPool / slot_size);
+        const int ncpad = shp.ncpad, ksj = shp.ksj, kst = shp.kst, kdir = shp.kdir;
+        const int slot_size = shp.slot_size, nslot = shp.nslot;
         const int slot = lane / ncpad, sl = lane - slot * ncpad;
         double* Ek = pool + (slot < nslot ? slot : 0) * slot_size;
         double* Rb0 = Ek + 3 * kdir;
@@ -292,8 +328,8 @@ __global__ __launch_bounds__(kHBlock) void hess2e_kernel(HessArgs g, const HItem
                 cart_exps(C.L, ic, ec);
                 cart_exps(D.L, id, ed);
             }
-            double acc[kHPairs];
-            sfor<kHPairs>([&](auto op) { acc[op] = 0.0; });
+            double acc[kHPairs];  // per coordinate pair: sum of pref x integral over the item
+            for (int op = 0; op < kHPairs; ++op) acc[op] = 0.0;
             for (int bp = w.b0; bp < w.b1; ++bp) {
                 const int i = bp / B.np, j = bp % B.np;
                 const double al = g.alpha[A.po + i], be = g.alpha[B.po + j], p = al + be;
@@ -386,12 +422,13 @@ __global__ __launch_bounds__(kHBlock) void hess2e_kernel(HessArgs g, const HItem
                                 }
                             }
                         }
-                        sfor<kHPairs>([&](auto op) {
-                            constexpr PairDirs pd = pair_dirs(op);
-                            const double* cx = cv[0][pd.k[0]];
-                            const double* cy = cv[1][pd.k[1]];
-                            const double* cz = cv[2][pd.k[2]];
-                            const int lx = len[0][pd.k[0]], ly = len[1][pd.k[1]], lz = len[2][pd.k[2]];
+#pragma unroll 1
+                        for (int op = 0; op < kHPairs; ++op) {
+                            const int kx = kPairs.k[op][0], ky = kPairs.k[op][1], kz = kPairs.k[op][2];
+                            const double* cx = cv[0][kx];
+                            const double* cy = cv[1][ky];
+                            const double* cz = cv[2][kz];
+                            const int lx = len[0][kx], ly = len[1][ky], lz = len[2][kz];
                             double I = 0.0;
                             for (int S = 0; S < lx; ++S) {
                                 double sy = 0.0;
@@ -403,7 +440,7 @@ __global__ __launch_bounds__(kHBlock) void hess2e_kernel(HessArgs g, const HItem
                                 I += cx[S] * sy;
                             }
                             acc[op] += pref * I;
-                        });
+                        }
                     }
                     __syncthreads();  // the slots' tables are rewritten by the next round
                 }
@@ -416,18 +453,19 @@ __global__ __launch_bounds__(kHBlock) void hess2e_kernel(HessArgs g, const HItem
                 gC = deg * 2.0 * g.Dc[fa * n + fb] * g.Dc[fc * n + fd];
                 gX = -deg * 0.5 * (g.Dc[fa * n + fc] * g.Dc[fb * n + fd] + g.Dc[fa * n + fd] * g.Dc[fb * n + fc]);
             }
-            sfor<kHPairs>([&](auto op) {
-                constexpr PairDirs pd = pair_dirs(op);
+#pragma unroll 1
+            for (int op = 0; op < kHPairs; ++op) {
+                const int x1 = kPairs.x1[op], x2 = kPairs.x2[op];
                 const double c = wave_sum(gC * acc[op]), x = wave_sum(gX * acc[op]);
                 if (lane == 0) {
-                    M[0][pd.x1 * 9 + pd.x2] += c;
-                    M[1][pd.x1 * 9 + pd.x2] += x;
-                    if (pd.x1 != pd.x2) {
-                        M[0][pd.x2 * 9 + pd.x1] += c;
-                        M[1][pd.x2 * 9 + pd.x1] += x;
+                    M[0][x1 * 9 + x2] += c;
+                    M[1][x1 * 9 + x2] += x;
+                    if (x1 != x2) {
+                        M[0][x2 * 9 + x1] += c;
+                        M[1][x2 * 9 + x1] += x;
                     }
                 }
-            });
+            }
         }
         __syncthreads();
         const int atoms[4] = {A.atom, B.atom, C.atom, D.atom};
@@ -518,9 +556,10 @@ int hfg_hessian_2e(hfg_ctx* c, const double* D, double* HC, double* HX) {
     g.prim_cut = c->opt.integral_tol / 1e4;
     g.HC = dH;
     g.HX = dH + nh;
-    // work items of at most kItemPrims primitive quartets, in whole bra primitive pairs, grouped by
-    // the order N = la + lb + lc + ld + 2 (neighbouring workgroups then take items of similar cost)
-    constexpr int kItemPrims = 512;
+    // work items of at most kItemRounds wavefront rounds (component chunks x rounds of ket primitive
+    // pairs over the slots x bra primitive pairs, as the kernel lays them out), in whole bra
+    // primitive pairs, grouped by the order N = la + lb + lc + ld + 2
+    constexpr int kItemRounds = 32;
     std::vector<std::vector<hfg::HItem>> by_n(hfg::kHN + 1);
     const int np = (int)pairs.size();
     for (int P = 0; P < np; ++P)
@@ -528,7 +567,10 @@ int hfg_hessian_2e(hfg_ctx* c, const double* D, double* HC, double* HX) {
             const int a = pairs[P].x, b = pairs[P].y, cc = pairs[Q].x, d = pairs[Q].y;
             const int N = c->L[a] + c->L[b] + c->L[cc] + c->L[d] + 2;
             const int nbra = c->nprim[a] * c->nprim[b], nket = c->nprim[cc] * c->nprim[d];
-            const int per = std::max(1, kItemPrims / nket);
+            const hfg::SlotShape sh = hfg::slot_shape(c->L[a], c->L[b], c->L[cc], c->L[d]);
+            const int chunks = (sh.ncomp + hfg::kHBlock - 1) / hfg::kHBlock;
+            const int rounds_per_bra = chunks * ((nket + sh.nslot - 1) / sh.nslot);
+            const int per = std::max(1, kItemRounds / rounds_per_bra);
             for (int b0 = 0; b0 < nbra; b0 += per) by_n[N].push_back(hfg::HItem{P, Q, b0, std::min(nbra, b0 + per)});
         }
     std::vector<int4> packed;
