@@ -302,6 +302,54 @@ def eri_export_figures():
     return out
 
 
+def _hessian_case(nwater, seed=1):
+    """(H2O)n / cc-pVDZ from the first n waters of the (H2O)8 geometry, and a fixed density
+    D = C C^T of 5 n random orbitals (the two-electron Hessian is a function of any density)."""
+    Z, X = H.Molecule.from_xyz(os.path.join(ROOT, "tests", "golden", "h2o_8.xyz"), "cc-pvdz").geometry()
+    mol = H.Molecule.from_arrays(Z[:3 * nwater], X[:3 * nwater], "cc-pvdz")
+    C = np.random.default_rng(seed).standard_normal((mol.nbf, 5 * nwater)) / np.sqrt(mol.nbf)
+    return mol, C @ C.T
+
+
+def hessian_figures():
+    """The two-electron Hessian at a fixed density (hfg_hessian_2e) of (H2O)2 and (H2O)4 /
+    cc-pVDZ on the GPU, second call of each (the first sets up the context's device copies). The
+    oracle's time for (H2O)2 is in cpu_baseline["hessian_2e"]."""
+    out = {"hessian_workload": "(H2O)4 / cc-pVDZ (96 functions, 1176 shell pairs, 692076 shell quartets), "
+                               "two-electron Hessian at a fixed density, HC and HX"}
+    for nw in (2, 4):
+        mol, D = _hessian_case(nw)
+        ctx = H.GpuContext(mol)
+        ctx.hessian_2e(D)
+        t = time.perf_counter()
+        hc, hx = ctx.hessian_2e(D)
+        sec = time.perf_counter() - t
+        ctx.close()
+        out[f"hessian_h2o{nw}_gpu_seconds"] = sec
+        if nw == 2:
+            out["_hessian_h2o2"] = (hc, hx)  # for the parity figure of the cpu_baseline leg
+        log(f"two-electron Hessian (H2O){nw}: {sec:.3f} s on the GPU")
+    return out
+
+
+def cpu_hessian(gpu_h2o2=None):
+    """The oracle's two-electron Hessian of (H2O)2 / cc-pVDZ on the host cores, and the largest
+    difference from the GPU's (HC, HX) of the same case when given."""
+    import oracle
+    mol, D = _hessian_case(2)
+    o = oracle.Oracle(mol.shells(), mol.geometry()[1], mol.pure, 1e-14, screen=False)
+    t = time.perf_counter()
+    hc, hx = o.hess2e(D)
+    sec = time.perf_counter() - t
+    threads = int(os.environ.get("OMP_NUM_THREADS") or 0) or min(os.cpu_count() or 1, 16)
+    out = {"workload": "(H2O)2 / cc-pVDZ two-electron Hessian at a fixed density", "seconds": sec,
+           "threads": threads, "kind": "port"}
+    if gpu_h2o2 is not None:
+        out["max_abs_diff_gpu_vs_oracle"] = float(max(np.abs(gpu_h2o2[0] - hc).max(), np.abs(gpu_h2o2[1] - hx).max()))
+    log(f"two-electron Hessian (H2O)2: oracle {sec:.1f} s on {threads} threads")
+    return out
+
+
 def cpu_eri_export():
     """The oracle's stored-ERI build of the (H2O)8 / cc-pVDZ vector on the host cores (the CPU
     side of the ERI export figure)."""
@@ -464,13 +512,16 @@ def main():
         if not args.no_scf:
             ex.update(oneint_figures(mol, scf["D"], scf["F"]))
         ex.update(eri_export_figures())
+        ex.update(hessian_figures())
         if args.batch_molecules > 0:
             ex.update(batch_figures(args.batch_molecules, args.solo_sample))
         out["extra"] = ex
+    gpu_h2o2 = out.get("extra", {}).pop("_hessian_h2o2", None)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(mol)
         if not args.no_extra:
             out["cpu_baseline"]["eri_export"] = cpu_eri_export()
+            out["cpu_baseline"]["hessian_2e"] = cpu_hessian(gpu_h2o2)
     print(json.dumps(out), flush=True)
 
 
