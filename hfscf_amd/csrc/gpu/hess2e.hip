@@ -429,12 +429,26 @@ __global__ __launch_bounds__(kHBlock) __attribute__((amdgpu_num_vgpr(128))) void
                             const double* cy = cv[1][ky];
                             const double* cz = cv[2][kz];
                             const int lx = len[0][kx], ly = len[1][ky], lz = len[2][kz];
+                            double czr[kHS];  // the innermost factor in registers
+#pragma unroll
+                            for (int V = 0; V < kHS; ++V) czr[V] = V < lz ? cz[V] : 0.0;
                             double I = 0.0;
                             for (int S = 0; S < lx; ++S) {
                                 double sy = 0.0;
                                 for (int U = 0; U < ly; ++U) {
+                                    // ridx(S, U, V) stepped in V: the step is (s+1)(s+2)/2 + q + 2 at
+                                    // s = S + U + V, q = U + V, and grows by s + 3 per step
+                                    const int s0 = S + U;
+                                    int idx = ridx(S, U, 0), ds = (s0 + 1) * (s0 + 2) / 2 + U + 2;
                                     double sz = 0.0;
-                                    for (int V = 0; V < lz; ++V) sz += cz[V] * R[ridx(S, U, V)];
+#pragma unroll
+                                    for (int V = 0; V < kHS; ++V) {
+                                        if (V < lz) {
+                                            sz += czr[V] * R[idx];
+                                            idx += ds;
+                                            ds += s0 + V + 3;
+                                        }
+                                    }
                                     sy += cy[U] * sz;
                                 }
                                 I += cx[S] * sy;
