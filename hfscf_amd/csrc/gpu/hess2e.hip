@@ -587,13 +587,10 @@ int hfg_hessian_2e(hfg_ctx* c, const double* D, double* HC, double* HX) {
             const int per = std::max(1, kItemRounds / rounds_per_bra);
             for (int b0 = 0; b0 < nbra; b0 += per) by_n[N].push_back(hfg::HItem{P, Q, b0, std::min(nbra, b0 + per)});
         }
+    // heaviest classes first, so that the launch does not end on a few long items
     std::vector<int4> packed;
-    std::vector<size_t> first(hfg::kHN + 2, 0);
-    for (int N = 0; N <= hfg::kHN; ++N) {
-        first[N] = packed.size();
+    for (int N = hfg::kHN; N >= 0; --N)
         for (const hfg::HItem& it : by_n[N]) packed.push_back(make_int4(it.P, it.Q, it.b0, it.b1));
-    }
-    first[hfg::kHN + 1] = packed.size();
     HFG_CHECK(c, c->d_hess_items.upload(packed, c->stream));
     const hfg::HItem* d_items = reinterpret_cast<const hfg::HItem*>(c->d_hess_items.p);
     static_assert(sizeof(hfg::HItem) == sizeof(int4), "work items travel as int4");
