@@ -1786,6 +1786,9 @@ int hfg_eri_vector(hfg_ctx* c, double* out, long long count) {
     a.dmax = c->d_dmax.p;
     a.dtol = 0.0;
     a.eri_out = c->d_eri_out.p;
+    // the class kernels may spill: they run on the device's shared scratch stream, after the
+    // vector's zeroing on the context's stream
+    HFG_CHECK(c, hipStreamSynchronize(c->stream));
     for (size_t k = 0; k < c->classes.size(); ++k) {
         // the FP64 per-lane kernel of the class, or its cooperative kernel where the J/K builds run
         // that one (the per-shell items: a class on the shell-group lists has them too)
@@ -1794,12 +1797,13 @@ int hfg_eri_vector(hfg_ctx* c, double* out, long long count) {
         b.items = c->d_items.p + cw.dev_off;
         b.queue = c->d_queue.p + k;
         const int n_items = (int)cw.items.size();
-        HFG_CHECK(c, launch_zero_ints(b.queue, 1, c->stream));
+        HFG_CHECK(c, launch_zero_ints(b.queue, 1, c->heavy));
         if (cw.coop)
-            HFG_CHECK(c, launch_jk_coop(b, cw.coop_args, cw.coop_lds[0], 0, n_items, LaunchShape{0, 0}, c->stream));
+            HFG_CHECK(c, launch_jk_coop(b, cw.coop_args, cw.coop_lds[0], 0, n_items, LaunchShape{0, 0}, c->heavy));
         else
-            HFG_CHECK(c, launch_jk(cw.la, cw.lb, cw.lc, cw.ld, c->pure, 0, b, n_items, LaunchShape{0, 0}, c->stream));
+            HFG_CHECK(c, launch_jk(cw.la, cw.lb, cw.lc, cw.ld, c->pure, 0, b, n_items, LaunchShape{0, 0}, c->heavy));
     }
+    HFG_CHECK(c, hipStreamSynchronize(c->heavy));
     HFG_CHECK(c, hipMemcpyAsync(out, c->d_eri_out.p, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HFG_CHECK(c, hipStreamSynchronize(c->stream));
     c->d_eri_out.release();  // the vector grows as N^4; the context keeps no copy

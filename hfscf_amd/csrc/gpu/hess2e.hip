@@ -594,10 +594,13 @@ int hfg_hessian_2e(hfg_ctx* c, const double* D, double* HC, double* HX) {
     HFG_CHECK(c, c->d_hess_items.upload(packed, c->stream));
     const hfg::HItem* d_items = reinterpret_cast<const hfg::HItem*>(c->d_hess_items.p);
     static_assert(sizeof(hfg::HItem) == sizeof(int4), "work items travel as int4");
+    // the kernel spills to scratch: it runs on the device's shared scratch stream, after the uploads
+    HFG_CHECK(c, hipStreamSynchronize(c->stream));
     if (!packed.empty())
         hipLaunchKernelGGL(hfg::hess2e_kernel, dim3((unsigned)std::min<size_t>(packed.size(), 1 << 16)), dim3(hfg::kHBlock),
-                           0, c->stream, g, d_items, (int)packed.size());
+                           0, c->heavy, g, d_items, (int)packed.size());
     HFG_CHECK(c, hipGetLastError());
+    HFG_CHECK(c, hipStreamSynchronize(c->heavy));
     HFG_CHECK(c, hipMemcpyAsync(HC, g.HC, nh * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HFG_CHECK(c, hipMemcpyAsync(HX, g.HX, nh * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HFG_CHECK(c, hipStreamSynchronize(c->stream));

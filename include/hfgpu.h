@@ -21,7 +21,8 @@
  *   - A context owns its main stream, its side streams (six by default) and all its device
  *     memory; it is not thread-safe. The class kernels that spill (private segments of several
  *     KB per lane) run on one more stream that every context on a device shares, one launch at
- *     a time, so that a single queue holds their scratch; contexts used from several threads on
+ *     a time, so that a single queue holds their scratch (hfg_eri_vector and hfg_hessian_2e
+ *     launch there too); contexts used from several threads on
  *     one device therefore serialise those launches, and a HIP graph (hfg_options.graph) never
  *     captures them (they are launched directly at each replay).
  */
