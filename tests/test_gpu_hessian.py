@@ -56,3 +56,17 @@ def test_hessian_2e_h2o4_invariants():
         print(f"(H2O)4 {name}: max {np.abs(M).max():.2e} asym {sym:.1e} translation {trans:.1e}", flush=True)
         assert sym < 1e-9 and trans < 1e-8
 
+
+
+def test_hessian_2e_rejects_bad_arguments():
+    Z, X = basis_ref.read_xyz(golden("h2o.xyz"))
+    mol = H.Molecule.from_arrays(Z, X, "sto-3g")
+    ctx = H.GpuContext(mol)
+    with pytest.raises(ValueError):
+        ctx.hessian_2e(np.zeros((mol.nbf + 1, mol.nbf + 1)))
+    n3 = 3 * len(Z)
+    # null outputs are refused by the entry point itself, with the context's message
+    rc = H.hfgpu().hfg_hessian_2e(ctx._h, H._ptr(np.zeros((mol.nbf, mol.nbf))), None, None)
+    assert rc != 0
+    hc, hx = ctx.hessian_2e(np.zeros((mol.nbf, mol.nbf)))  # a zero density gives zero Hessians
+    assert hc.shape == (n3, n3) and not hc.any() and not hx.any()
