@@ -6,19 +6,22 @@
 // (src/integrals/hfscf_oserideriv2qq.cpp:28-749, hfscf_oserideriv2pq.cpp:23-799) and contracts
 // them afterwards; here they are contracted with the density as they are made and never stored.
 //
-// One wavefront per canonical shell quartet (ab|cd), a >= b, c >= d, (ab) >= (cd). Integrals by
+// One wavefront per work item: a canonical shell quartet (ab|cd), a >= b, c >= d, (ab) >= (cd),
+// and a range of its bra primitive pairs, at most 32 wavefront rounds. Integrals by
 // McMurchie-Davidson: per primitive quartet the 1-D Hermite coefficients E^{ij}_t of the bra
 // (i <= la + 2, j <= lb + 2) and ket (i <= lc + 2, j <= ld) in LDS, and the Hermite Coulomb
-// integrals R_tuv(pq/(p+q), P - Q) up to order la + lb + lc + ld + 2, made level by level by the
-// wavefront. A derivative of a Gaussian with respect to its centre is d/dA_x |a> = 2 alpha |a+1x>
+// integrals R_tuv(pq/(p+q), P - Q) up to order la + lb + lc + ld + 2, made level by level. A class
+// with at most 32 Cartesian component quartets runs several ket primitive pairs at once, each in a
+// slot of lanes with its own ket and R tables. A derivative of a Gaussian with respect to its centre is d/dA_x |a> = 2 alpha |a+1x>
 // - a_x |a-1x>, applied once or twice to the centres A, B, C (D from translational invariance:
 // d/dD = -(d/dA + d/dB + d/dC)); it acts on one direction's Hermite coefficients only, so each
 // lane, holding one Cartesian component quartet, forms per direction the convolution over the
 // bra and ket Hermite indices of the shifted coefficients (ten bra/ket operator combinations of
 // total order <= 2) and sums them against R for all 45 coordinate pairs of (A, B, C) x (x, y, z).
-// The lane adds each pair's integral times its Coulomb and exchange densities into 90
-// accumulators; at the quartet's end the wavefront reduces them, expands the 9 x 9 block to the
-// 12 x 12 block of the four centres and adds it to the atoms' rows and columns.
+// The lane sums each pair's integrals over the item's primitive quartets (45 accumulators); at the
+// item's end they are weighted by the component quartet's Coulomb and exchange densities, reduced
+// over the wavefront, expanded from the 9 x 9 block to the 12 x 12 block of the four centres and
+// added to the atoms' rows and columns.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
