@@ -79,11 +79,17 @@ __host__ __device__ constexpr int rsize(int n) { return (n + 1) * (n + 2) * (n +
 constexpr int kComboBra[10] = {0, 1, 2, 3, 4, 5, 0, 1, 2, 0};
 constexpr int kComboKet[10] = {0, 0, 0, 0, 0, 0, 1, 1, 1, 2};
 constexpr int kBraOrder[6] = {0, 1, 1, 2, 2, 2};
+// the combinations the (A, P, C) coordinates use: none, A, AA, C, AC, CC
+constexpr int kUsedCombo[6] = {0, 1, 3, 6, 7, 9};
 
 // Combination per direction of coordinate pair `op` (X1 <= X2, X = 3 centre + direction, in the
-// order (0,0) (0,1) .. (0,8) (1,1) .. (8,8)).
+// order (0,0) (0,1) .. (0,8) (1,1) .. (8,8)), with the centres taken as A, P, C: P is the bra's
+// Hermite centre, whose derivative raises the Hermite index (d/dP_x Lambda_t = Lambda_{t+1}), so
+// a P derivative shifts the R index of its direction by one (p[d]) and needs no coefficients of
+// its own; d/dB = d/dP - d/dA then gives the B derivatives (the 9 x 9 block is mapped at the end).
 struct PairDirs {
     int k[3];
+    int p[3];
     int x1, x2;
 };
 constexpr PairDirs pair_dirs(int op) {
@@ -95,31 +101,35 @@ constexpr PairDirs pair_dirs(int op) {
                 X2 = b;
             }
     int bc[3] = {0, 0, 0}, kc[3] = {0, 0, 0};
+    PairDirs r{{0, 0, 0}, {0, 0, 0}, X1, X2};
     const int xs[2] = {X1, X2};
     for (int q = 0; q < 2; ++q) {
         const int c = xs[q] / 3, d = xs[q] % 3;
         if (c == 0)
-            bc[d] = bc[d] == 0 ? 1 : (bc[d] == 1 ? 3 : 4);
+            bc[d] = bc[d] == 0 ? 1 : 3;
         else if (c == 1)
-            bc[d] = bc[d] == 0 ? 2 : (bc[d] == 2 ? 5 : 4);
+            r.p[d] += 1;
         else
             kc[d] += 1;
     }
-    PairDirs r{{0, 0, 0}, X1, X2};
-    for (int d = 0; d < 3; ++d) r.k[d] = kc[d] == 0 ? bc[d] : (kc[d] == 2 ? 9 : (bc[d] == 0 ? 6 : (bc[d] == 1 ? 7 : 8)));
+    for (int d = 0; d < 3; ++d) r.k[d] = kc[d] == 0 ? bc[d] : (kc[d] == 2 ? 9 : (bc[d] == 0 ? 6 : 7));
     return r;
 }
 
 // pair_dirs of every coordinate pair, for loops over the pairs that are not unrolled.
 struct PairTab {
     unsigned char k[kHPairs][3];
+    unsigned char p[kHPairs][3];
     unsigned char x1[kHPairs], x2[kHPairs];
 };
 constexpr PairTab make_pairs() {
     PairTab t{};
     for (int op = 0; op < kHPairs; ++op) {
         const PairDirs pd = pair_dirs(op);
-        for (int d = 0; d < 3; ++d) t.k[op][d] = (unsigned char)pd.k[d];
+        for (int d = 0; d < 3; ++d) {
+            t.k[op][d] = (unsigned char)pd.k[d];
+            t.p[op][d] = (unsigned char)pd.p[d];
+        }
         t.x1[op] = (unsigned char)pd.x1;
         t.x2[op] = (unsigned char)pd.x2;
     }
@@ -281,7 +291,7 @@ __global__ __launch_bounds__(kHBlock) __attribute__((amdgpu_num_vgpr(128))) void
                                                                                           int nitems) {
     __shared__ double Eb[3 * kHE * kHE * kHT];
     __shared__ double pool[kHPool];  // per slot: ket E^{ij}_t (3 directions), R ping-pong, scaled F_m
-    __shared__ double M[2][81];
+    __shared__ double M[2][81];  // the item's 9 x 9 blocks over (A, P, C), then (A, B, C), x (x, y, z)
     const int lane = threadIdx.x;
     for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
         const HItem w = items[it];
@@ -408,7 +418,8 @@ __global__ __launch_bounds__(kHBlock) __attribute__((amdgpu_num_vgpr(128))) void
                         for (int d = 0; d < 3; ++d) {
                             const double* Ebd = Eb + d * kHE * kHE * kHT;
                             const double* Ekd = Ek + d * kdir;
-                            for (int cb = 0; cb < 10; ++cb) {
+                            for (int u = 0; u < 6; ++u) {
+                                const int cb = kUsedCombo[u];
                                 const int bcode = kComboBra[cb], kcode = kComboKet[cb];
                                 const int nt = ea[d] + eb[d] + kBraOrder[bcode] + 1;
                                 const int nk = ec[d] + ed[d] + kcode + 1;
@@ -428,6 +439,7 @@ __global__ __launch_bounds__(kHBlock) __attribute__((amdgpu_num_vgpr(128))) void
 #pragma unroll 1
                         for (int op = 0; op < kHPairs; ++op) {
                             const int kx = kPairs.k[op][0], ky = kPairs.k[op][1], kz = kPairs.k[op][2];
+                            const int px = kPairs.p[op][0], py = kPairs.p[op][1], pz = kPairs.p[op][2];
                             const double* cx = cv[0][kx];
                             const double* cy = cv[1][ky];
                             const double* cz = cv[2][kz];
@@ -439,17 +451,18 @@ __global__ __launch_bounds__(kHBlock) __attribute__((amdgpu_num_vgpr(128))) void
                             for (int S = 0; S < lx; ++S) {
                                 double sy = 0.0;
                                 for (int U = 0; U < ly; ++U) {
-                                    // ridx(S, U, V) stepped in V: the step is (s+1)(s+2)/2 + q + 2 at
-                                    // s = S + U + V, q = U + V, and grows by s + 3 per step
-                                    const int s0 = S + U;
-                                    int idx = ridx(S, U, 0), ds = (s0 + 1) * (s0 + 2) / 2 + U + 2;
+                                    // ridx(t, u, v), t = S + px, u = U + py, v = V + pz, stepped in V:
+                                    // the step is (s+1)(s+2)/2 + (u+v) + 2 at s = t + u + v, and grows
+                                    // by s + 3 per step
+                                    const int t0 = S + px, u0 = U + py, s0 = t0 + u0 + pz;
+                                    int idx = ridx(t0, u0, pz), ds = (s0 + 1) * (s0 + 2) / 2 + u0 + pz + 2;
                                     double sz = 0.0;
 #pragma unroll
                                     for (int V = 0; V < kHS; ++V) {
                                         if (V < lz) {
                                             sz += czr[V] * R[idx];
                                             idx += ds;
-                                            ds += s0 + V + 3;
+                                            ds += s0 + V + 3;  // s = s0 + V at this step
                                         }
                                     }
                                     sy += cy[U] * sz;
@@ -484,6 +497,30 @@ __global__ __launch_bounds__(kHBlock) __attribute__((amdgpu_num_vgpr(128))) void
                 }
             }
         }
+        __syncthreads();
+        // (A, P, C) -> (A, B, C): d/dB = d/dP - d/dA in each direction (in place, through registers)
+        double mv[(162 + kHBlock - 1) / kHBlock];
+        for (int r = 0, e = lane; e < 162; ++r, e += kHBlock) {
+            const int m = e / 81, x1 = (e % 81) / 9, x2 = e % 9;
+            double v = 0.0;
+            for (int q1 = 0; q1 < 2; ++q1) {
+                // row x1 of the map: A -> A; B -> P - A; C -> C
+                const int c1 = x1 / 3;
+                if (q1 == 1 && c1 != 1) continue;
+                const int y1 = c1 == 1 ? (q1 == 0 ? x1 : x1 - 3) : x1;
+                const double w1 = (c1 == 1 && q1 == 1) ? -1.0 : 1.0;
+                for (int q2 = 0; q2 < 2; ++q2) {
+                    const int c2 = x2 / 3;
+                    if (q2 == 1 && c2 != 1) continue;
+                    const int y2 = c2 == 1 ? (q2 == 0 ? x2 : x2 - 3) : x2;
+                    const double w2 = (c2 == 1 && q2 == 1) ? -1.0 : 1.0;
+                    v += w1 * w2 * M[m][y1 * 9 + y2];
+                }
+            }
+            mv[r] = v;
+        }
+        __syncthreads();
+        for (int r = 0, e = lane; e < 162; ++r, e += kHBlock) M[e / 81][e % 81] = mv[r];
         __syncthreads();
         const int atoms[4] = {A.atom, B.atom, C.atom, D.atom};
         finish_item(g, atoms, M);
