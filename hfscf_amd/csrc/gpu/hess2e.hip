@@ -13,11 +13,13 @@
 // integrals R_tuv(pq/(p+q), P - Q) up to order la + lb + lc + ld + 2, made level by level. A class
 // with at most 32 Cartesian component quartets runs several ket primitive pairs at once, each in a
 // slot of lanes with its own ket and R tables. A derivative of a Gaussian with respect to its centre is d/dA_x |a> = 2 alpha |a+1x>
-// - a_x |a-1x>, applied once or twice to the centres A, B, C (D from translational invariance:
-// d/dD = -(d/dA + d/dB + d/dC)); it acts on one direction's Hermite coefficients only, so each
-// lane, holding one Cartesian component quartet, forms per direction the convolution over the
-// bra and ket Hermite indices of the shifted coefficients (ten bra/ket operator combinations of
-// total order <= 2) and sums them against R for all 45 coordinate pairs of (A, B, C) x (x, y, z).
+// - a_x |a-1x>, applied once or twice to the centres A and C; the bra's B derivatives come from
+// d/dB = d/dP - d/dA with P the bra's Hermite centre (d/dP raises the R index by one), and D from
+// translational invariance, d/dD = -(d/dA + d/dB + d/dC). A derivative acts on one direction's
+// Hermite coefficients only, so each lane, holding one Cartesian component quartet, forms per
+// direction the convolution over the bra and ket Hermite indices of the shifted coefficients (six
+// operator combinations) and sums them against R for all 45 coordinate pairs of (A, P, C) x
+// (x, y, z).
 // The lane sums each pair's integrals over the item's primitive quartets (45 accumulators); at the
 // item's end they are weighted by the component quartet's Coulomb and exchange densities, reduced
 // over the wavefront, expanded from the 9 x 9 block to the 12 x 12 block of the four centres and
